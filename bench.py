@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X variant on the HBM-bound FP64 workload.
+
+Workload (one GPU, synthetic seeded inputs, seed 42, resident in HBM):
+  Stream_TRIAD       N = 2^28 doubles (a, b, c: 2 GiB each)
+  Stream_DOT         N = 2^28
+  Algorithm_REDUCE_SUM  N = 2^28, x uniform in [-1,1)
+  Algorithm_SCAN     N = 2^28 (exclusive), x uniform in [-1,1)
+  Polybench_JACOBI_2D  8192 x 8192 grid, interior uniform [0,1), zero
+                     boundary, 50 time steps (100 sweeps)
+One step is one rep of each kernel, in that order, on one HIP stream.
+
+`value` is the aggregate algorithmic throughput of a step (the suite's
+bytes-per-rep of every kernel summed, divided by the step time).  The
+roofline object describes the dominant kernel (the JACOBI_2D sweep): its
+algorithmic bytes per launch over its average launch time measured with HIP
+events on the launch stream.  The CPU baseline times the oracle's
+Base_OpenMP restatement of the same kernels on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 42
+N_STREAM = 1 << 28
+N_GRID = 8192
+TSTEPS = 50
+
+
+def algorithmic_bytes():
+    n, N = N_STREAM, N_GRID
+    sweep = 8 * (N * N - 4) + 8 * (N - 2) * (N - 2)  # POLYBENCH_JACOBI_2D.cpp:37-42 per sweep
+    return {
+        "Stream_TRIAD": 24 * n,                   # TRIAD.cpp:30-32
+        "Stream_DOT": 8 + 16 * n + 8,             # DOT.cpp:31-35
+        "Algorithm_REDUCE_SUM": 8 * (1 + n) + 8,  # REDUCE_SUM.cpp:31-33
+        "Algorithm_SCAN": 16 * n,                 # SCAN.cpp:31-32
+        "Polybench_JACOBI_2D": 2 * TSTEPS * sweep,
+        "_jacobi_sweep": sweep,
+    }
+
+
+def read_traffic(kernel_key: str):
+    """HBM bytes per dispatch for a kernel from the committed counter summary
+    (profiles/*counters*.txt lines `kernel=<key> hbm_bytes_per_dispatch=<B>`)."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if "counters" not in fn or not fn.endswith(".txt"):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            m = re.match(r"\s*kernel=(\S+)\s+hbm_bytes_per_dispatch=([0-9.eE+]+)", line)
+            if m and m.group(1) == kernel_key:
+                return float(m.group(2))
+    return None
+
+
+def gpu_run(steps: int, warmup: int):
+    from rajaperf_amd import abi
+    arch = abi.init(0)
+    L = abi.lib()
+    n, N = N_STREAM, N_GRID
+    s = None  # null stream: every launch and event below is on it
+    B = abi.DeviceBuffer
+    a, b, c = B(8 * n), B(8 * n), B(8 * n)
+    x, y = B(8 * n), B(8 * n)
+    gA, gB = B(8 * N * N), B(8 * N * N)
+    res = B(16)
+    L.rpx_fill_uniform_f64(b.ptr, n, SEED, 1, 0.0, 1.0, s)
+    L.rpx_fill_uniform_f64(c.ptr, n, SEED, 2, 0.0, 1.0, s)
+    L.rpx_fill_const_f64(a.ptr, n, 0.0, s)
+    L.rpx_fill_uniform_f64(x.ptr, n, SEED, 0, -1.0, 1.0, s)
+    L.rpx_fill_grid_f64(gA.ptr, N, SEED, 0, 0.0, 1.0, 0.0, s)
+    L.rpx_fill_grid_f64(gB.ptr, N, SEED, 1, 0.0, 1.0, 0.0, s)
+    L.rpx_fill_const_f64(res.ptr, 2, 0.0, s)
+    rsb = L.rpx_reduce_scratch_bytes(n, None)
+    rscr = B(rsb)
+    L.rpx_reduce_scratch_init(rscr.ptr, rsb, s)
+    ssb = L.rpx_scan_scratch_bytes(n, None)
+    sscr = B(ssb)
+    L.rpx_scan_scratch_init(sscr.ptr, ssb, s)
+    abi.synchronize()
+
+    names = ["Stream_TRIAD", "Stream_DOT", "Algorithm_REDUCE_SUM", "Algorithm_SCAN", "Polybench_JACOBI_2D"]
+    # One event before each kernel of each timed step plus one at the end.
+    evs = [[abi.Event() for _ in range(len(names) + 1)] for _ in range(steps)]
+
+    def step(ev):
+        if ev:
+            ev[0].record(s)
+        L.rpx_triad_f64(a.ptr, b.ptr, c.ptr, 0.75, n, None, s)
+        if ev:
+            ev[1].record(s)
+        L.rpx_dot_f64(a.ptr, b.ptr, n, res.ptr, 1, rscr.ptr, rsb, None, s)
+        if ev:
+            ev[2].record(s)
+        L.rpx_reduce_sum_f64(x.ptr, n, res.ptr + 8, rscr.ptr, rsb, None, s)
+        if ev:
+            ev[3].record(s)
+        L.rpx_scan_exclusive_f64(x.ptr, y.ptr, n, sscr.ptr, ssb, None, s)
+        if ev:
+            ev[4].record(s)
+        L.rpx_jacobi2d_f64(gA.ptr, gB.ptr, N, TSTEPS, None, s)
+        if ev:
+            ev[5].record(s)
+
+    for _ in range(warmup):
+        step(None)
+    abi.synchronize()
+    t_start, t_stop = abi.Event(), abi.Event()
+    wall0 = time.perf_counter()
+    t_start.record(s)
+    for k in range(steps):
+        step(evs[k])
+    t_stop.record(s)
+    abi.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_ms(t_stop)
+    per_kernel_ms = {k: 0.0 for k in names}
+    for ev in evs:
+        for i, k in enumerate(names):
+            per_kernel_ms[k] += ev[i].elapsed_ms(ev[i + 1])
+    if L.rpx_scan_scratch_status(sscr.ptr) != 0:
+        raise RuntimeError("scan look-back wait expired")
+    return arch, total_ms, per_kernel_ms, wall
+
+
+def cpu_baseline():
+    """Oracle Base_OpenMP on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.build()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    n = 1 << 25
+    Ng = 4096
+    ts = 5
+    runs = [("Stream_TRIAD", dict(size=n)), ("Stream_DOT", dict(size=n)),
+            ("Algorithm_REDUCE_SUM", dict(size=n)), ("Algorithm_SCAN", dict(size=n)),
+            ("Polybench_JACOBI_2D", dict(size=(Ng - 2) ** 2, tsteps=ts))]
+    total_bytes, total_s = 0, 0.0
+    for k, kw in runs:
+        r = orc.run(k, "Base_OpenMP", input_mode=1, seed=SEED, checkrun=2, nthreads=threads, **kw)
+        total_bytes += r["bytes_per_rep"] * r["reps"]
+        total_s += r["time_s"]
+    return {"value": total_bytes / total_s / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"oracle Base_OpenMP ({threads} threads), 2 reps each of TRIAD/DOT/REDUCE_SUM/SCAN "
+                      f"at N=2^25 and JACOBI_2D {Ng}x{Ng} x {ts} tsteps, seeded inputs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the kernels are single-device; multi-GPU sharding is a design note).", file=sys.stderr)
+        sys.exit(2)
+
+    arch, total_ms, per_ms, wall = gpu_run(args.steps, args.warmup)
+    ab = algorithmic_bytes()
+    step_bytes = sum(v for k, v in ab.items() if not k.startswith("_"))
+    ms_per_step = total_ms / args.steps
+    value = step_bytes * args.steps / (total_ms / 1e3) / 1e9
+
+    sweep_ms = per_ms["Polybench_JACOBI_2D"] / (args.steps * 2 * TSTEPS)
+    achieved = ab["_jacobi_sweep"] / (sweep_ms / 1e3) / 1e9
+    traffic = read_traffic("jacobi2d_sweep_8192")
+    kernels = {}
+    for k, ms in per_ms.items():
+        t = ms / args.steps
+        gbs = ab[k] / (t / 1e3) / 1e9
+        kernels[k] = {"ms_per_rep": round(t, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+
+    out = {
+        "metric": "per-kernel GB/s or GFLOP/s vs roofline; mixed vs FP64 speedup at stated error",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 42, generated on device)",
+        "config": {"workload": "HBM-bound plain FP64: TRIAD/DOT/REDUCE_SUM/SCAN at N=2^28, "
+                               "JACOBI_2D 8192x8192 x 50 tsteps",
+                   "n_stream": N_STREAM, "jacobi_n": N_GRID, "jacobi_tsteps": TSTEPS,
+                   "variant": "MI355X_Native", "arch": arch},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "jacobi2d sweep 8192x8192", "avg_launch_ms": round(sweep_ms, 5)},
+        "kernels": kernels,
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,247 @@
+/*
+ * rajaperf_mi355x.h -- C ABI of the MI355X-native execution variant.
+ *
+ * One entry point per kernel of the RAJA Performance Suite subset this
+ * library accelerates.  Each one replaces the device work done inside the
+ * timed rep loop of a `run<Backend>Variant` method of the suite's
+ * KernelBase plugin API (src/common/KernelBase.hpp:451-487): the suite's
+ * setUp/updateChecksum/tearDown stay on the caller's side.
+ *
+ * Conventions (all entry points):
+ *   - every array pointer is a DEVICE pointer owned by the caller, except
+ *     where a parameter name ends in `_host`;
+ *   - no allocation and no host synchronisation inside a compute entry
+ *     point; work is queued on `stream` (an hipStream_t passed as void*,
+ *     NULL = the null stream) and completes asynchronously;
+ *   - the return value is RPX_SUCCESS (0), a negative RPX_ERR_* code for
+ *     argument errors, or -(hipError_t) for a HIP runtime failure;
+ *   - calls on different streams with disjoint buffers may run concurrently;
+ *     scratch buffers must not be shared by calls in flight at the same time;
+ *   - `tune` may be NULL: the library then uses its own tuned defaults.
+ *
+ * Floating-point policy: element-wise kernels (COPY/MUL/ADD/TRIAD),
+ * JACOBI_2D and the FP64 FEM operators evaluate every expression in the
+ * association of the suite's Base_Seq loop bodies with no fused
+ * multiply-add contraction, so their results equal the Base_Seq result bit
+ * for bit.  Reductions, scans and GEMM change the summation order and are
+ * compared within a stated tolerance.  SORT is value-exact.
+ */
+#ifndef RAJAPERF_MI355X_H
+#define RAJAPERF_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPX_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define RPX_SUCCESS                  0
+#define RPX_ERR_INVALID_ARGUMENT  -1001 /* null pointer, negative size, ...  */
+#define RPX_ERR_SCRATCH_TOO_SMALL -1002 /* scratch_bytes below the query     */
+#define RPX_ERR_UNSUPPORTED       -1003 /* shape outside the compiled range  */
+#define RPX_ERR_NO_DEVICE         -1004 /* no gfx950 device visible          */
+#define RPX_ERR_WAIT_EXPIRED      -1005 /* a bounded device wait gave up     */
+
+/* ---- tuning ------------------------------------------------------------ */
+/* Replaces the suite's compile-time block-size / items-per-thread lists
+ * (src/common/GPUUtils.hpp:146-179, rajaperf_config.hpp.in:121-128).
+ * Zero fields mean "library default". */
+typedef struct rpx_tuning {
+  int32_t block_size;        /* work-items per workgroup                    */
+  int32_t items_per_thread;  /* elements per work-item per grid step        */
+  int32_t grid_size;         /* workgroups; 0 = derived from the size       */
+  int32_t flags;             /* RPX_TUNE_* bits                             */
+} rpx_tuning;
+
+#define RPX_TUNE_NONTEMPORAL_STORE 0x1 /* streaming stores bypass the cache */
+#define RPX_TUNE_NAIVE             0x2 /* GEMM: one-thread-per-C comparator */
+
+/* ---- library / runtime support ---------------------------------------- *
+ * Counterparts of the suite's HIP data utilities
+ * (src/common/HipDataUtils.hpp:141-255) and of the device synchronisation
+ * in KernelBase::synchronize (src/common/KernelBase.hpp:271-294). */
+int         rpx_abi_version(void);
+const char* rpx_status_string(int status);
+/* Selects device `device` and checks that it is a gfx950 part.
+ * `arch_out` (optional, host) receives the gcnArchName. */
+int rpx_init(int device, char* arch_out, size_t arch_len);
+int rpx_device_count(int* count_host);
+int rpx_device_malloc(void** ptr_host, size_t bytes);
+int rpx_device_free(void* ptr);
+int rpx_host_malloc_pinned(void** ptr_host, size_t bytes);
+int rpx_host_free_pinned(void* ptr);
+int rpx_memcpy_h2d(void* dst, const void* src_host, size_t bytes);
+int rpx_memcpy_d2h(void* dst_host, const void* src, size_t bytes);
+int rpx_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int rpx_stream_create(void** stream_host);
+int rpx_stream_destroy(void* stream);
+int rpx_stream_synchronize(void* stream);
+int rpx_device_synchronize(void);
+int rpx_event_create(void** event_host);
+int rpx_event_destroy(void* event);
+int rpx_event_record(void* event, void* stream);
+/* Milliseconds between two recorded events (synchronises on `stop`). */
+int rpx_event_elapsed_ms(float* ms_host, void* start, void* stop);
+
+/* ---- input generation -------------------------------------------------- *
+ * Seeded input mode: x[i] = lo + (hi-lo)*u_i with u_i = (s_i >> 11)*2^-53
+ * and s_i the i-th output of SplitMix64 started from
+ * state0 = mix64(seed ^ (stream_id * 0xD1B54A32D192ED03)).  Evaluated with
+ * the same rounding on host and device.  Replaces the host-side formula
+ * initialisers (src/common/DataUtils.cpp:477-595) for large inputs so they
+ * need no host-to-device copy. */
+int rpx_fill_uniform_f64(double* x, int64_t n, uint64_t seed,
+                         uint64_t stream_id, double lo, double hi,
+                         void* stream);
+/* N x N grid: interior points seeded as rpx_fill_uniform_f64 over the flat
+ * index j + i*N, boundary rows/columns set to `boundary`. */
+int rpx_fill_grid_f64(double* x, int64_t N, uint64_t seed, uint64_t stream_id,
+                      double lo, double hi, double boundary, void* stream);
+/* Symmetric tensor field [e][s][q3] (s < sym, sym = 6 or 9): diagonal
+ * components uniform in [0.5,2], off-diagonal in [-0.25,0.25], from the same
+ * SplitMix64 sequence over the flat index. */
+int rpx_fill_sym_tensor_f64(double* x, int64_t ne, int q3, int sym,
+                            uint64_t seed, uint64_t stream_id, void* stream);
+/* x[i] = val (src/common/DataUtils.cpp:518-528 initDataConst). */
+int rpx_fill_const_f64(double* x, int64_t n, double val, void* stream);
+/* Suite formula x[i] = factor*(i+1.1)/(i+1.12345)
+ * (src/common/DataUtils.cpp:504-513 initData(Real_ptr)). */
+int rpx_fill_suite_formula_f64(double* x, int64_t n, double factor,
+                               void* stream);
+
+/* ---- stream group ------------------------------------------------------ */
+/* c[i] = a[i]             src/stream/COPY.hpp:24-25, COPY-Hip.cpp          */
+int rpx_copy_f64(double* c, const double* a, int64_t n,
+                 const rpx_tuning* tune, void* stream);
+/* b[i] = alpha*c[i]       src/stream/MUL.hpp:25-26, MUL-Hip.cpp            */
+int rpx_mul_f64(double* b, const double* c, double alpha, int64_t n,
+                const rpx_tuning* tune, void* stream);
+/* c[i] = a[i] + b[i]      src/stream/ADD.hpp:25-26, ADD-Hip.cpp            */
+int rpx_add_f64(double* c, const double* a, const double* b, int64_t n,
+                const rpx_tuning* tune, void* stream);
+/* a[i] = b[i] + alpha*c[i]  src/stream/TRIAD.hpp:26-27, TRIAD-Hip.cpp:24-61 */
+int rpx_triad_f64(double* a, const double* b, const double* c, double alpha,
+                  int64_t n, const rpx_tuning* tune, void* stream);
+
+/* ---- reductions -------------------------------------------------------- *
+ * Deterministic: a fixed grid writes one FP64 partial per workgroup into
+ * `scratch`; the last workgroup to finish sums the partials in index order.
+ * Results are identical from run to run for a fixed (n, tune).
+ * `scratch` must be zeroed once before its first use
+ * (rpx_reduce_scratch_init); every call leaves it ready for the next. */
+size_t rpx_reduce_scratch_bytes(int64_t n, const rpx_tuning* tune);
+int rpx_reduce_scratch_init(void* scratch, size_t scratch_bytes,
+                            void* stream);
+/* dot = sum a[i]*b[i]; *result = dot, or *result += dot when `accumulate`
+ * (src/stream/DOT.hpp:24-25; the suite's per-rep `m_dot += dot`,
+ * DOT-Seq.cpp:39-45, DOT-Hip.cpp:28-94).  `result` is a device scalar. */
+int rpx_dot_f64(const double* a, const double* b, int64_t n, double* result,
+                int accumulate, void* scratch, size_t scratch_bytes,
+                const rpx_tuning* tune, void* stream);
+/* *result = sum x[i]  (src/algorithm/REDUCE_SUM.hpp:31-32,
+ * REDUCE_SUM-Seq.cpp:34-51, REDUCE_SUM-Hip.cpp:36-198). */
+int rpx_reduce_sum_f64(const double* x, int64_t n, double* result,
+                       void* scratch, size_t scratch_bytes,
+                       const rpx_tuning* tune, void* stream);
+
+/* ---- scan -------------------------------------------------------------- *
+ * Exclusive prefix sum y[i] = x[0]+...+x[i-1], y[0] = 0
+ * (src/algorithm/SCAN.hpp:27-32, SCAN-Seq.cpp:31-45, SCAN-Hip.cpp:39-326).
+ * Single pass with decoupled look-back; tiles take ticketed ids so a tile
+ * only waits on tiles that were scheduled before it.  x and y may alias. */
+size_t rpx_scan_scratch_bytes(int64_t n, const rpx_tuning* tune);
+/* Zeroes a scan scratch buffer; required once before its first use.  Each
+ * call then leaves the buffer ready for the next (epoch-tagged flags). */
+int rpx_scan_scratch_init(void* scratch, size_t scratch_bytes, void* stream);
+/* Synchronously reads the scratch's error word: RPX_ERR_WAIT_EXPIRED if a
+ * look-back wait of an earlier call gave up, else RPX_SUCCESS. */
+int rpx_scan_scratch_status(const void* scratch);
+int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n,
+                           void* scratch, size_t scratch_bytes,
+                           const rpx_tuning* tune, void* stream);
+/* Segmented form: segment s covers [offsets[s], offsets[s+1]) and is scanned
+ * independently; `offsets_host` is a host array of nseg+1 ascending values. */
+size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host,
+                                        int64_t nseg);
+int rpx_scan_exclusive_segmented_f64(const double* x, double* y,
+                                     const int64_t* offsets_host,
+                                     int64_t nseg, void* scratch,
+                                     size_t scratch_bytes, void* stream);
+
+/* ---- sort -------------------------------------------------------------- *
+ * Ascending sort of FP64 keys, value-identical to std::sort
+ * (src/algorithm/SORT.hpp:18-25, SORT-Seq.cpp:31-42, SORT-Hip.cpp:35-43).
+ * -0.0 orders before +0.0; NaNs order after +inf (sign bit clear) or before
+ * -inf (sign bit set).  Segment s covers [offsets[s], offsets[s+1]) with
+ * `offsets_host` a host array; each segment is sorted independently, in
+ * place.  The suite's layout (reps segments of length n) is
+ * offsets[s] = s*n. */
+size_t rpx_sort_segmented_scratch_bytes(const int64_t* offsets_host,
+                                        int64_t nseg);
+int rpx_sort_segmented_f64(double* x, const int64_t* offsets_host,
+                           int64_t nseg, void* scratch, size_t scratch_bytes,
+                           void* stream);
+
+/* ---- polybench --------------------------------------------------------- */
+/* One Jacobi sweep dst[i][j] = 0.2*(src[c]+src[w]+src[e]+src[s]+src[n]) on
+ * the interior 1..N-2 of an N x N row-major grid; the boundary of dst is not
+ * written (src/polybench/POLYBENCH_JACOBI_2D.hpp:41-45). */
+int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N,
+                           const rpx_tuning* tune, void* stream);
+/* `tsteps` time steps of the two-sweep A->B->A update
+ * (POLYBENCH_JACOBI_2D-Seq.cpp:30-54, -Hip.cpp:27-114). */
+int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
+                     const rpx_tuning* tune, void* stream);
+
+/* GEMM with the suite's semantics (src/polybench/POLYBENCH_GEMM.hpp:12-50):
+ *   C[i][j] = sum_k (alpha*A[i][k])*B[k][j]
+ * `beta` is accepted for interface fidelity: the suite scales C by beta and
+ * then overwrites it, so beta never reaches the result.  A is ni x nk, B is
+ * nk x nj, C is ni x nj, all row-major.  FP64 on the MFMA f64 units;
+ * RPX_TUNE_NAIVE selects the one-thread-per-element comparator. */
+int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha,
+                 double beta, int64_t ni, int64_t nj, int64_t nk,
+                 const rpx_tuning* tune, void* stream);
+
+/* Mixed-precision GEMM, same semantics: A and B are split into `nslices`
+ * error-free fixed-point slices, slice products run on MFMA with exact
+ * integer accumulation, and the result is recombined in FP64.  nslices in
+ * [1, 8]; 0 = default (7, error below 1e-12 relative to |alpha||A||B|). */
+size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk,
+                                    int nslices);
+int rpx_gemm_mixed_f64(double* C, const double* A, const double* B,
+                       double alpha, double beta, int64_t ni, int64_t nj,
+                       int64_t nk, int nslices, void* scratch,
+                       size_t scratch_bytes, void* stream);
+
+/* ---- apps (partial-assembly finite elements) -------------------------- */
+/* Mass operator action Y_e += Bt'^3 (D_e .* (B^3 X_e)) for NE hex elements
+ * (src/apps/MASS3DPA.hpp:158-339, MASS3DPA-Seq.cpp:29-94).
+ * B is read as B[q + q1d*d]; Bt is read LITERALLY as Bt[q + d1d*d], the
+ * suite's indexing (MASS3DPA.hpp:175).  X, Y: [e][dz][dy][dx];
+ * D: [e][qz][qy][qx].  Supported: 1 <= d1d <= 8, d1d <= q1d <= 9. */
+int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D,
+                     const double* X, double* Y, int64_t NE, int d1d,
+                     int q1d, const rpx_tuning* tune, void* stream);
+
+/* Diffusion operator action with the suite's half-storage basis tables
+ * (src/apps/DIFFUSION3DPA.hpp:216-458, DIFFUSION3DPA-Seq.cpp:28-110).
+ * Basis/dBasis read as [q + q1d*d]; D: [e][s][qz][qy][qx] with 6 symmetric
+ * components (11,12,13,22,23,33) when `symmetric`, 9 otherwise.  The shared
+ * B/G table is filled in the Base_Seq write order, so bases that break the
+ * half-storage symmetry give the Base_Seq result.  Requires q1d = d1d+1,
+ * 1 <= d1d <= 8. */
+int rpx_diffusion3dpa_f64(const double* Basis, const double* dBasis,
+                          const double* D, const double* X, double* Y,
+                          int64_t NE, int d1d, int q1d, int symmetric,
+                          const rpx_tuning* tune, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAJAPERF_MI355X_H */

@@ -1,0 +1,488 @@
+// MASS3DPA and DIFFUSION3DPA: partial-assembly operator actions on hex
+// elements, sum-factorised, FP64.
+//
+// Reference: src/apps/MASS3DPA.hpp:158-339 with MASS3DPA-Seq.cpp:29-94, and
+// src/apps/DIFFUSION3DPA.hpp:216-458 with DIFFUSION3DPA-Seq.cpp:28-110.  The
+// suite's HIP variants run one element per workgroup of (Q1D,Q1D[,Q1D])
+// work-items with 8-9 barriers (MASS3DPA-Hip.cpp:25-111,
+// DIFFUSION3DPA-Hip.cpp:25-134).
+//
+// MI355X design: a workgroup of 256 work-items owns E elements.  Each of the
+// six contraction stages maps one work-item to one 1D "pencil" of one element
+// (e.g. the x-contraction to the (e,dz,dy) pencils), keeps the pencil in
+// registers and writes the stage result to LDS; stages are separated by
+// workgroup barriers, so a workgroup takes 6 barriers per E elements.  X and
+// the quadrature data are read once with coalesced pencil loads, Y is
+// updated in place.  The contraction order of every stage and the
+// association of every sum are those of the Base_Seq loops and no
+// multiply-add is fused, so results are bit-identical to Base_Seq.
+//
+// Stage order (forward x, y, z; backward x, y, z):
+//   P1 (e,dz,dy): DDQ[dz][dy][qx] = sum_dx X[dz][dy][dx] B(qx,dx)
+//   P2 (e,dz,qx): DQQ[dz][qy][qx] = sum_dy DDQ[dz][dy][qx] B(qy,dy)
+//   P3 (e,qy,qx): QQQ[qz][qy][qx] = (sum_dz DQQ[dz][qy][qx] B(qz,dz)) * D
+//   P4 (e,qz,qy): QQD[qz][qy][dx] = sum_qx QQQ[qz][qy][qx] Bt'(dx,qx)
+//   P5 (e,qz,dx): QDD[qz][dy][dx] = sum_qy QQD[qz][qy][dx] Bt'(dy,qy)
+//   P6 (e,dy,dx): Y[dz][dy][dx]  += sum_qz QDD[qz][dy][dx] Bt'(dz,qz)
+// with B(q,d) = B[q + Q*d] and the suite's literal Bt'(d,q) = Bt[q + D*d]
+// (MASS3DPA.hpp:175).  DIFFUSION3DPA carries three components per stage.
+#include "rpx_common.hpp"
+
+using namespace rpx;
+
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kLdsBudget = 48 * 1024;
+
+template <int D, int Q>
+struct MassShape {
+  static constexpr int kA = Q * Q * Q;  // DDQ / QQQ / QDD per element
+  static constexpr int kB = D * Q * Q;  // DQQ / QQD per element
+  static constexpr int kBytes = 8 * (kA + kB);
+  static constexpr int E0 = kLdsBudget / kBytes;
+  static constexpr int E = E0 < 1 ? 1 : (E0 > 64 ? 64 : E0);
+};
+
+template <int D, int Q>
+__global__ __launch_bounds__(kBS) void mass3dpa_kernel(const double* __restrict__ Bg,
+                                                       const double* __restrict__ Btg,
+                                                       const double* __restrict__ Dg,
+                                                       const double* __restrict__ Xg, double* __restrict__ Yg,
+                                                       int64_t NE) {
+  using S = MassShape<D, Q>;
+  constexpr int E = S::E;
+  constexpr int D2 = D * D, D3 = D * D * D, Q2 = Q * Q, Q3 = Q * Q * Q;
+  __shared__ double bufA[E * S::kA];
+  __shared__ double bufB[E * S::kB];
+  __shared__ double sB[Q * D];   // sB[q*D + d]  = B(q,d)
+  __shared__ double sBt[D * Q];  // sBt[d*Q + q] = Bt'(d,q) = Bt[q + D*d]
+
+  const int tid = threadIdx.x;
+  const int64_t e0 = int64_t(blockIdx.x) * E;
+  const int ne = int((NE - e0) < E ? (NE - e0) : E);
+
+  for (int t = tid; t < Q * D; t += kBS) {
+    const int q = t / D, d = t % D;
+    sB[t] = Bg[q + Q * d];
+    const int d2 = t / Q, q2 = t % Q;
+    sBt[t] = Btg[q2 + D * d2];
+  }
+  __syncthreads();
+
+  // P1: x-contraction of X pencils.
+  for (int w = tid; w < ne * D2; w += kBS) {
+    const int e = w / D2, r = w % D2, dz = r / D, dy = r % D;
+    const double* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
+    double x[D];
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) x[dx] = xp[dx];
+    double* out = bufA + e * S::kA + (dz * D + dy) * Q;
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      double u = 0.0;
+#pragma unroll
+      for (int dx = 0; dx < D; ++dx) u += x[dx] * sB[qx * D + dx];
+      out[qx] = u;
+    }
+  }
+  __syncthreads();
+  // P2: y-contraction.
+  for (int w = tid; w < ne * D * Q; w += kBS) {
+    const int e = w / (D * Q), r = w % (D * Q), dz = r / Q, qx = r % Q;
+    const double* in = bufA + e * S::kA + dz * D * Q + qx;  // DDQ[dz][dy][qx], stride Q in dy
+    double v[D];
+#pragma unroll
+    for (int dy = 0; dy < D; ++dy) v[dy] = in[dy * Q];
+    double* out = bufB + e * S::kB + dz * Q2 + qx;  // DQQ[dz][qy][qx], stride Q in qy
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) {
+      double u = 0.0;
+#pragma unroll
+      for (int dy = 0; dy < D; ++dy) u += v[dy] * sB[qy * D + dy];
+      out[qy * Q] = u;
+    }
+  }
+  __syncthreads();
+  // P3: z-contraction and the quadrature-point scaling.
+  for (int w = tid; w < ne * Q2; w += kBS) {
+    const int e = w / Q2, r = w % Q2;  // r = qy*Q + qx
+    const double* in = bufB + e * S::kB + r;  // DQQ[dz][qy][qx], stride Q2 in dz
+    double v[D];
+#pragma unroll
+    for (int dz = 0; dz < D; ++dz) v[dz] = in[dz * Q2];
+    const double* dp = Dg + (e0 + e) * Q3 + r;
+    double* out = bufA + e * S::kA + r;  // QQQ[qz][qy][qx], stride Q2 in qz
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) {
+      double u = 0.0;
+#pragma unroll
+      for (int dz = 0; dz < D; ++dz) u += v[dz] * sB[qz * D + dz];
+      out[qz * Q2] = u * dp[qz * Q2];
+    }
+  }
+  __syncthreads();
+  // P4: backward x-contraction.
+  for (int w = tid; w < ne * Q2; w += kBS) {
+    const int e = w / Q2, r = w % Q2;  // r = qz*Q + qy
+    const double* in = bufA + e * S::kA + r * Q;  // QQQ[qz][qy][qx], contiguous in qx
+    double v[Q];
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) v[qx] = in[qx];
+    double* out = bufB + e * S::kB + r * D;  // QQD[qz][qy][dx]
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) {
+      double u = 0.0;
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) u += v[qx] * sBt[dx * Q + qx];
+      out[dx] = u;
+    }
+  }
+  __syncthreads();
+  // P5: backward y-contraction.
+  for (int w = tid; w < ne * Q * D; w += kBS) {
+    const int e = w / (Q * D), r = w % (Q * D), qz = r / D, dx = r % D;
+    const double* in = bufB + e * S::kB + qz * Q * D + dx;  // QQD[qz][qy][dx], stride D in qy
+    double v[Q];
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) v[qy] = in[qy * D];
+    double* out = bufA + e * S::kA + qz * D2 + dx;  // QDD[qz][dy][dx], stride D in dy
+#pragma unroll
+    for (int dy = 0; dy < D; ++dy) {
+      double u = 0.0;
+#pragma unroll
+      for (int qy = 0; qy < Q; ++qy) u += v[qy] * sBt[dy * Q + qy];
+      out[dy * D] = u;
+    }
+  }
+  __syncthreads();
+  // P6: backward z-contraction, accumulate into Y.
+  for (int w = tid; w < ne * D2; w += kBS) {
+    const int e = w / D2, r = w % D2;  // r = dy*D + dx
+    const double* in = bufA + e * S::kA + r;  // QDD[qz][dy][dx], stride D2 in qz
+    double v[Q];
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) v[qz] = in[qz * D2];
+    double* yp = Yg + (e0 + e) * D3 + r;
+#pragma unroll
+    for (int dz = 0; dz < D; ++dz) {
+      double u = 0.0;
+#pragma unroll
+      for (int qz = 0; qz < Q; ++qz) u += v[qz] * sBt[dz * Q + qz];
+      yp[dz * D2] += u;
+    }
+  }
+}
+
+// ---- DIFFUSION3DPA ----------------------------------------------------------
+
+// Effective basis tables of the suite's half-storage scheme
+// (DIFFUSION3DPA.hpp:243-268, DIFFUSION3DPA_2 / _6).  With Q = D+1 the B and
+// G slots of the shared table are disjoint, every slot has exactly two
+// writers -- the reader's own (q,d) pair and its mirror (Q-1-q, D-1-d) -- and
+// the Base_Seq loops write them in (d outer, q inner) order, identically for
+// the forward (B, G) and the backward (Bt, Gt) tables.  The value a reader
+// (q,d) sees is therefore that of whichever of the two writers comes later in
+// that order; G values carry the writer's sign and are multiplied by the
+// reader's sign on use.
+__device__ __forceinline__ void diffusion_tables(int q, int d, int Q, int D, const double* b, const double* g,
+                                                 double* Beff, double* Geff) {
+  auto key = [&](int qq, int dd) { return dd * Q + qq; };
+  auto sgn = [](int qq, int dd) { return qq <= dd ? -1.0 : 1.0; };
+  // B slot read by (q,d) and its two writers.
+  const int bi = q <= d ? q : Q - 1 - q, bj = q <= d ? d : D - 1 - d;
+  const int mi = Q - 1 - bi, mj = D - 1 - bj;  // mirror writer
+  const bool self_last = key(bi, bj) > key(mi, mj);
+  *Beff = self_last ? b[bi + Q * bj] : b[mi + Q * mj];
+  // G slot read by (q,d): writers (k,l) and its mirror.
+  const int gk = q <= d ? Q - 1 - q : q, gl = q <= d ? D - 1 - d : d;
+  const int nk = Q - 1 - gk, nl = D - 1 - gl;
+  const bool g_self_last = key(gk, gl) > key(nk, nl);
+  const double stored = g_self_last ? g[gk + Q * gl] * sgn(gk, gl) : g[nk + Q * nl] * sgn(nk, nl);
+  *Geff = stored * sgn(q, d);
+}
+
+template <int D, int Q>
+struct DiffShape {
+  static constexpr int kA = 3 * Q * Q * Q;
+  static constexpr int kB = 3 * D * Q * Q;
+  static constexpr int kBytes = 8 * (kA + kB);
+  static constexpr int E0 = kLdsBudget / kBytes;
+  static constexpr int E = E0 < 1 ? 1 : (E0 > 64 ? 64 : E0);
+};
+
+template <int D, int Q, bool SYMM>
+__global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __restrict__ Bg,
+                                                            const double* __restrict__ Gg,
+                                                            const double* __restrict__ Dg,
+                                                            const double* __restrict__ Xg,
+                                                            double* __restrict__ Yg, int64_t NE) {
+  using S = DiffShape<D, Q>;
+  constexpr int E = S::E;
+  constexpr int D2 = D * D, D3 = D * D * D, Q2 = Q * Q, Q3 = Q * Q * Q;
+  constexpr int NC = SYMM ? 6 : 9;
+  constexpr int A1 = Q3, B1 = D * Q2;  // per-component strides within an element
+  __shared__ double bufA[E * S::kA];
+  __shared__ double bufB[E * S::kB];
+  __shared__ double sB[Q * D];  // sB[q*D + d] = effective B(q,d)
+  __shared__ double sG[Q * D];  // sG[q*D + d] = effective G(q,d) with the reader's sign
+
+  const int tid = threadIdx.x;
+  const int64_t e0 = int64_t(blockIdx.x) * E;
+  const int ne = int((NE - e0) < E ? (NE - e0) : E);
+
+  for (int t = tid; t < Q * D; t += kBS) {
+    const int q = t / D, d = t % D;
+    double be, ge;
+    diffusion_tables(q, d, Q, D, Bg, Gg, &be, &ge);
+    sB[t] = be;
+    sG[t] = ge;
+  }
+  __syncthreads();
+
+  // P1 (DIFFUSION3DPA_3): DDQ0 = sum X*B, DDQ1 = sum X*G over dx.
+  for (int w = tid; w < ne * D2; w += kBS) {
+    const int e = w / D2, r = w % D2, dz = r / D, dy = r % D;
+    const double* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
+    double x[D];
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) x[dx] = xp[dx];
+    double* out = bufA + e * S::kA + (dz * D + dy) * Q;
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      double u = 0.0, v = 0.0;
+#pragma unroll
+      for (int dx = 0; dx < D; ++dx) {
+        u += x[dx] * sB[qx * D + dx];
+        v += x[dx] * sG[qx * D + dx];
+      }
+      out[qx] = u;
+      out[A1 + qx] = v;
+    }
+  }
+  __syncthreads();
+  // P2 (DIFFUSION3DPA_4): DQQ0 = sum DDQ1*B, DQQ1 = sum DDQ0*G, DQQ2 = sum DDQ0*B over dy.
+  for (int w = tid; w < ne * D * Q; w += kBS) {
+    const int e = w / (D * Q), r = w % (D * Q), dz = r / Q, qx = r % Q;
+    const double* in = bufA + e * S::kA + dz * D * Q + qx;
+    double a0[D], a1[D];
+#pragma unroll
+    for (int dy = 0; dy < D; ++dy) {
+      a0[dy] = in[dy * Q];
+      a1[dy] = in[A1 + dy * Q];
+    }
+    double* out = bufB + e * S::kB + dz * Q2 + qx;
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) {
+      double u = 0.0, v = 0.0, ww = 0.0;
+#pragma unroll
+      for (int dy = 0; dy < D; ++dy) {
+        u += a1[dy] * sB[qy * D + dy];
+        v += a0[dy] * sG[qy * D + dy];
+        ww += a0[dy] * sB[qy * D + dy];
+      }
+      out[qy * Q] = u;
+      out[B1 + qy * Q] = v;
+      out[2 * B1 + qy * Q] = ww;
+    }
+  }
+  __syncthreads();
+  // P3 (DIFFUSION3DPA_5): gradient at the point, times the symmetric tensor.
+  for (int w = tid; w < ne * Q2; w += kBS) {
+    const int e = w / Q2, r = w % Q2;
+    const double* in = bufB + e * S::kB + r;
+    double c0[D], c1[D], c2[D];
+#pragma unroll
+    for (int dz = 0; dz < D; ++dz) {
+      c0[dz] = in[dz * Q2];
+      c1[dz] = in[B1 + dz * Q2];
+      c2[dz] = in[2 * B1 + dz * Q2];
+    }
+    const double* dp = Dg + (e0 + e) * (NC * Q3) + r;
+    double* out = bufA + e * S::kA + r;
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) {
+      double u = 0.0, v = 0.0, ww = 0.0;
+#pragma unroll
+      for (int dz = 0; dz < D; ++dz) {
+        u += c0[dz] * sB[qz * D + dz];
+        v += c1[dz] * sB[qz * D + dz];
+        ww += c2[dz] * sG[qz * D + dz];
+      }
+      const double* o = dp + qz * Q2;
+      const double O11 = o[0];
+      const double O12 = o[Q3];
+      const double O13 = o[2 * Q3];
+      const double O21 = SYMM ? O12 : o[3 * Q3];
+      const double O22 = SYMM ? o[3 * Q3] : o[4 * Q3];
+      const double O23 = SYMM ? o[4 * Q3] : o[5 * Q3];
+      const double O31 = SYMM ? O13 : o[6 * Q3];
+      const double O32 = SYMM ? O23 : o[7 * Q3];
+      const double O33 = SYMM ? o[5 * Q3] : o[8 * Q3];
+      out[qz * Q2] = ((O11 * u) + (O12 * v)) + (O13 * ww);
+      out[A1 + qz * Q2] = ((O21 * u) + (O22 * v)) + (O23 * ww);
+      out[2 * A1 + qz * Q2] = ((O31 * u) + (O32 * v)) + (O33 * ww);
+    }
+  }
+  __syncthreads();
+  // P4 (DIFFUSION3DPA_7): QQD0 = sum QQQ0*Gt, QQD1 = sum QQQ1*Bt, QQD2 = sum QQQ2*Bt over qx.
+  for (int w = tid; w < ne * Q2; w += kBS) {
+    const int e = w / Q2, r = w % Q2;  // r = qz*Q + qy
+    const double* in = bufA + e * S::kA + r * Q;
+    double c0[Q], c1[Q], c2[Q];
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      c0[qx] = in[qx];
+      c1[qx] = in[A1 + qx];
+      c2[qx] = in[2 * A1 + qx];
+    }
+    double* out = bufB + e * S::kB + r * D;
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) {
+      double u = 0.0, v = 0.0, ww = 0.0;
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        u += c0[qx] * sG[qx * D + dx];
+        v += c1[qx] * sB[qx * D + dx];
+        ww += c2[qx] * sB[qx * D + dx];
+      }
+      out[dx] = u;
+      out[B1 + dx] = v;
+      out[2 * B1 + dx] = ww;
+    }
+  }
+  __syncthreads();
+  // P5 (DIFFUSION3DPA_8): QDD0 = sum QQD0*Bt, QDD1 = sum QQD1*Gt, QDD2 = sum QQD2*Bt over qy.
+  for (int w = tid; w < ne * Q * D; w += kBS) {
+    const int e = w / (Q * D), r = w % (Q * D), qz = r / D, dx = r % D;
+    const double* in = bufB + e * S::kB + qz * Q * D + dx;
+    double c0[Q], c1[Q], c2[Q];
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) {
+      c0[qy] = in[qy * D];
+      c1[qy] = in[B1 + qy * D];
+      c2[qy] = in[2 * B1 + qy * D];
+    }
+    double* out = bufA + e * S::kA + qz * D2 + dx;
+#pragma unroll
+    for (int dy = 0; dy < D; ++dy) {
+      double u = 0.0, v = 0.0, ww = 0.0;
+#pragma unroll
+      for (int qy = 0; qy < Q; ++qy) {
+        u += c0[qy] * sB[qy * D + dy];
+        v += c1[qy] * sG[qy * D + dy];
+        ww += c2[qy] * sB[qy * D + dy];
+      }
+      out[dy * D] = u;
+      out[A1 + dy * D] = v;
+      out[2 * A1 + dy * D] = ww;
+    }
+  }
+  __syncthreads();
+  // P6 (DIFFUSION3DPA_9): sum over qz, Y += (u + v) + w.
+  for (int w = tid; w < ne * D2; w += kBS) {
+    const int e = w / D2, r = w % D2;
+    const double* in = bufA + e * S::kA + r;
+    double c0[Q], c1[Q], c2[Q];
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) {
+      c0[qz] = in[qz * D2];
+      c1[qz] = in[A1 + qz * D2];
+      c2[qz] = in[2 * A1 + qz * D2];
+    }
+    double* yp = Yg + (e0 + e) * D3 + r;
+#pragma unroll
+    for (int dz = 0; dz < D; ++dz) {
+      double u = 0.0, v = 0.0, ww = 0.0;
+#pragma unroll
+      for (int qz = 0; qz < Q; ++qz) {
+        u += c0[qz] * sB[qz * D + dz];
+        v += c1[qz] * sB[qz * D + dz];
+        ww += c2[qz] * sG[qz * D + dz];
+      }
+      yp[dz * D2] += (u + v) + ww;
+    }
+  }
+}
+
+template <int D, int Q>
+int launch_mass(const double* B, const double* Bt, const double* Dq, const double* X, double* Y, int64_t NE,
+                hipStream_t s) {
+  const int64_t grid = ceil_div(NE, MassShape<D, Q>::E);
+  if (grid > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  mass3dpa_kernel<D, Q><<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(B, Bt, Dq, X, Y, NE);
+  return launch_status();
+}
+
+template <int D>
+int mass_q(int q1d, const double* B, const double* Bt, const double* Dq, const double* X, double* Y, int64_t NE,
+           hipStream_t s) {
+  if (q1d == D) return launch_mass<D, D>(B, Bt, Dq, X, Y, NE, s);
+  if (q1d == D + 1) return launch_mass<D, D + 1>(B, Bt, Dq, X, Y, NE, s);
+  if constexpr (D + 2 <= 9) {
+    if (q1d == D + 2) return launch_mass<D, D + 2>(B, Bt, Dq, X, Y, NE, s);
+  }
+  return RPX_ERR_UNSUPPORTED;
+}
+
+template <int D, bool SYMM>
+int launch_diff(const double* B, const double* G, const double* Dq, const double* X, double* Y, int64_t NE,
+                hipStream_t s) {
+  constexpr int Q = D + 1;
+  const int64_t grid = ceil_div(NE, DiffShape<D, Q>::E);
+  if (grid > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  diffusion3dpa_kernel<D, Q, SYMM><<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(B, G, Dq, X, Y, NE);
+  return launch_status();
+}
+
+template <bool SYMM>
+int diff_d(int d1d, const double* B, const double* G, const double* Dq, const double* X, double* Y, int64_t NE,
+           hipStream_t s) {
+  switch (d1d) {
+    case 1: return launch_diff<1, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 2: return launch_diff<2, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 3: return launch_diff<3, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 4: return launch_diff<4, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 5: return launch_diff<5, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 6: return launch_diff<6, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 7: return launch_diff<7, SYMM>(B, G, Dq, X, Y, NE, s);
+    case 8: return launch_diff<8, SYMM>(B, G, Dq, X, Y, NE, s);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D, const double* X, double* Y, int64_t NE,
+                     int d1d, int q1d, const rpx_tuning* /*tune*/, void* stream) {
+  if (NE < 0) return RPX_ERR_INVALID_ARGUMENT;
+  if (NE == 0) return RPX_SUCCESS;
+  if (!B || !Bt || !D || !X || !Y) return RPX_ERR_INVALID_ARGUMENT;
+  hipStream_t s = as_stream(stream);
+  switch (d1d) {
+    case 1: return mass_q<1>(q1d, B, Bt, D, X, Y, NE, s);
+    case 2: return mass_q<2>(q1d, B, Bt, D, X, Y, NE, s);
+    case 3: return mass_q<3>(q1d, B, Bt, D, X, Y, NE, s);
+    case 4: return mass_q<4>(q1d, B, Bt, D, X, Y, NE, s);
+    case 5: return mass_q<5>(q1d, B, Bt, D, X, Y, NE, s);
+    case 6: return mass_q<6>(q1d, B, Bt, D, X, Y, NE, s);
+    case 7: return mass_q<7>(q1d, B, Bt, D, X, Y, NE, s);
+    case 8: return mass_q<8>(q1d, B, Bt, D, X, Y, NE, s);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+int rpx_diffusion3dpa_f64(const double* Basis, const double* dBasis, const double* D, const double* X, double* Y,
+                          int64_t NE, int d1d, int q1d, int symmetric, const rpx_tuning* /*tune*/, void* stream) {
+  if (NE < 0) return RPX_ERR_INVALID_ARGUMENT;
+  if (NE == 0) return RPX_SUCCESS;
+  if (!Basis || !dBasis || !D || !X || !Y) return RPX_ERR_INVALID_ARGUMENT;
+  if (q1d != d1d + 1) return RPX_ERR_UNSUPPORTED;
+  hipStream_t s = as_stream(stream);
+  return symmetric ? diff_d<true>(d1d, Basis, dBasis, D, X, Y, NE, s)
+                   : diff_d<false>(d1d, Basis, dBasis, D, X, Y, NE, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// GEMM (polybench): C[i][j] = sum_k (alpha*A[i][k]) * B[k][j].
+//
+// Reference: src/polybench/POLYBENCH_GEMM.hpp:12-50 (C *= beta, then C is
+// overwritten by the dot, so beta never reaches the result),
+// POLYBENCH_GEMM-Seq.cpp:30-50, POLYBENCH_GEMM-Hip.cpp:27-102 (one thread
+// per C element, no LDS, no matrix cores).
+//
+// Two FP64 paths:
+//  * gemm_naive_kernel: the one-thread-per-element shape of the suite's
+//    Base_HIP variant, kept as the "plain FP64 GPU" comparator;
+//  * gemm_mfma_f64_kernel: 128x128 workgroup tiles, four wavefronts of 64x64,
+//    each a 4x4 grid of v_mfma_f64_16x16x4f64 accumulators; A (pre-scaled by
+//    alpha, as the suite rounds alpha*A[i][k]) and B staged through LDS in
+//    16-deep K slabs with the next slab's global loads in flight during the
+//    current slab's MFMAs.  Tiles are mapped XCD-major so neighbouring tiles
+//    that share A rows / B columns run on the same XCD's L2.
+#include "rpx_common.hpp"
+
+using namespace rpx;
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ---- plain FP64 comparator ---------------------------------------------------
+__global__ __launch_bounds__(256) void gemm_naive_kernel(double* __restrict__ C, const double* __restrict__ A,
+                                                         const double* __restrict__ B, double alpha, int64_t ni,
+                                                         int64_t nj, int64_t nk) {
+  const int64_t j = int64_t(blockIdx.x) * 32 + (threadIdx.x & 31);
+  const int64_t i = int64_t(blockIdx.y) * 8 + (threadIdx.x >> 5);
+  if (i >= ni || j >= nj) return;
+  double dot = 0.0;
+  for (int64_t k = 0; k < nk; ++k) dot = fma(alpha * A[k + i * nk], B[j + k * nj], dot);
+  C[j + i * nj] = dot;
+}
+
+// ---- FP64 MFMA ---------------------------------------------------------------
+constexpr int kTM = 128, kTN = 128, kTK = 16;
+constexpr int kThreads = 256;
+constexpr int kLdsStride = kTM + 2;  // [k][m] layout, padded
+
+__device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned G) {
+  return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+}
+
+__global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restrict__ C,
+                                                                 const double* __restrict__ A,
+                                                                 const double* __restrict__ B, double alpha,
+                                                                 int64_t ni, int64_t nj, int64_t nk,
+                                                                 int tiles_n) {
+  __shared__ double As[2][kTK * kLdsStride];  // As[k][m]
+  __shared__ double Bs[2][kTK * kLdsStride];  // Bs[k][n]
+
+  const unsigned tile = xcd_major(blockIdx.x, gridDim.x);
+  const int64_t m0 = int64_t(tile / unsigned(tiles_n)) * kTM;
+  const int64_t n0 = int64_t(tile % unsigned(tiles_n)) * kTN;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 64;  // wave's row offset in the tile
+  const int wn = (wave & 1) * 64;   // wave's column offset
+
+  // Global load assignment: A slab 128 x 16 (row-major in A), B slab 16 x 128.
+  const int a_row = tid >> 1, a_k = (tid & 1) * 8;
+  const int b_k = tid >> 4, b_col = (tid & 15) * 8;
+
+  double ra[8], rb[8];
+  auto load_slab = [&](int64_t k0) {
+    const int64_t gi = m0 + a_row;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t gk = k0 + a_k + u;
+      ra[u] = (gi < ni && gk < nk) ? alpha * A[gi * nk + gk] : 0.0;
+    }
+    const int64_t gk = k0 + b_k;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t gj = n0 + b_col + u;
+      rb[u] = (gk < nk && gj < nj) ? B[gk * nj + gj] : 0.0;
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) As[buf][(a_k + u) * kLdsStride + a_row] = ra[u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) Bs[buf][b_k * kLdsStride + b_col + u] = rb[u];
+  };
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const int64_t nslabs = (nk + kTK - 1) / kTK;
+  load_slab(0);
+  store_slab(0);
+  __syncthreads();
+  const int fr = lane & 15;   // fragment row (A) / column (B)
+  const int fk = lane >> 4;   // fragment k within a 4-deep step
+  for (int64_t s = 0; s < nslabs; ++s) {
+    const int buf = int(s & 1);
+    if (s + 1 < nslabs) load_slab((s + 1) * kTK);
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk += 4) {
+      double af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[buf][(kk + fk) * kLdsStride + wm + i * 16 + fr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][(kk + fk) * kLdsStride + wn + j * 16 + fr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nslabs) {
+      store_slab(buf ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // v_mfma_f64_16x16x4f64 accumulator layout (measured on gfx950): element r
+  // of lane l is row 4*r + l/16, column l%16.
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = m0 + wm + i * 16 + 4 * r + (lane >> 4);
+        const int64_t gj = n0 + wn + j * 16 + (lane & 15);
+        if (gi < ni && gj < nj) C[gi * nj + gj] = acc[i][j][r];
+      }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha, double /*beta*/, int64_t ni,
+                 int64_t nj, int64_t nk, const rpx_tuning* tune, void* stream) {
+  if (ni < 0 || nj < 0 || nk < 0) return RPX_ERR_INVALID_ARGUMENT;
+  if (ni == 0 || nj == 0) return RPX_SUCCESS;
+  if (!C || (nk > 0 && (!A || !B))) return RPX_ERR_INVALID_ARGUMENT;
+  hipStream_t s = as_stream(stream);
+  if (tune_flags(tune) & RPX_TUNE_NAIVE) {
+    const int64_t gx = ceil_div(nj, 32), gy = ceil_div(ni, 8);
+    if (gx > INT32_MAX || gy > 65535) return RPX_ERR_UNSUPPORTED;
+    gemm_naive_kernel<<<dim3(unsigned(gx), unsigned(gy)), dim3(256), 0, s>>>(C, A, B, alpha, ni, nj, nk);
+    return launch_status();
+  }
+  const int64_t tm = ceil_div(ni, kTM), tn = ceil_div(nj, kTN);
+  if (tm * tn > INT32_MAX || tn > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  gemm_mfma_f64_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, A, B, alpha, ni, nj, nk, int(tn));
+  return launch_status();
+}
+
+}  // extern "C"

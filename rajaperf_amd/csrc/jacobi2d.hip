@@ -1,0 +1,167 @@
+// JACOBI_2D: 5-point Jacobi sweeps on an N x N row-major FP64 grid.
+//
+// Reference: src/polybench/POLYBENCH_JACOBI_2D.hpp:41-45 (bodies),
+// POLYBENCH_JACOBI_2D-Seq.cpp:30-54 (tsteps x two sweeps over the interior),
+// POLYBENCH_JACOBI_2D-Hip.cpp:27-114 (one thread per point, 32 x bs/32
+// blocks, no data reuse beyond the caches).
+//
+// MI355X design: a workgroup owns a tile of 512 columns x R rows.  Each
+// work-item holds a column pair as one 16-byte vector; it issues the loads of
+// all R+2 rows of its pair before any arithmetic (R+2 16-byte loads in flight
+// per work-item), takes its west/east neighbours from the adjacent lanes by
+// cross-lane shuffles (only the two edge lanes of a wavefront load a scalar
+// halo), and stores the R output rows as 16-byte vectors.  Tiles are
+// numbered so that each XCD walks a horizontal band of the grid top to
+// bottom: the two halo rows a tile shares with its vertical neighbour are
+// then read from the same XCD's L2.  Each output is
+// 0.2*((((c + w) + e) + s) + n), the Base_Seq association, so sweeps are
+// bit-identical to Base_Seq.
+#include "rpx_common.hpp"
+
+using namespace rpx;
+
+namespace {
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+constexpr int kBS = 256;             // work-items per tile row
+constexpr int kCols = 2 * kBS;       // columns per tile
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ double point(double c, double w, double e, double s, double n) {
+  return 0.2 * ((((c + w) + e) + s) + n);
+}
+
+// XCD-aware tile id: hardware block b runs on XCD b % 8; give XCD x the
+// contiguous range of logical tiles [x*G/8, (x+1)*G/8).
+__device__ __forceinline__ unsigned logical_tile(unsigned b, unsigned G) {
+  if (G % kXcds != 0) return b;
+  return (b % kXcds) * (G / kXcds) + b / kXcds;
+}
+
+template <int R>
+__global__ __launch_bounds__(kBS) void jacobi_vec_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                                                         int64_t N, int strips, int chunks) {
+  const unsigned tile = logical_tile(blockIdx.x, gridDim.x);
+  const int strip = int(tile % unsigned(strips));
+  const int chunk = int(tile / unsigned(strips));
+  const int lane = threadIdx.x & (kWave - 1);
+
+  const int64_t pair = int64_t(strip) * kBS + threadIdx.x;  // column pair index
+  const int64_t j0 = 2 * pair;                               // first column of the pair
+  const bool in_grid = j0 < N;
+  const int64_t r0 = 1 + int64_t(chunk) * R;                // first output row
+  int64_t rend = r0 + R;
+  if (rend > N - 1) rend = N - 1;
+  const int rows = int(rend - r0);
+
+  // Loads: rows r0-1 .. r0+R (clamped), the pair and the edge halos.
+  v2d v[R + 2];
+  double wh[R], eh[R];
+#pragma unroll
+  for (int k = 0; k < R + 2; ++k) {
+    v[k] = v2d{0.0, 0.0};
+    if (in_grid && k <= rows + 1) v[k] = *reinterpret_cast<const v2d*>(src + (r0 - 1 + k) * N + j0);
+  }
+  const bool left_edge = (lane == 0) && in_grid && j0 >= 1;
+  const bool right_edge = (lane == kWave - 1) && in_grid && j0 + 2 < N;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    wh[k] = 0.0;
+    eh[k] = 0.0;
+    if (k < rows) {
+      if (left_edge) wh[k] = src[(r0 + k) * N + j0 - 1];
+      if (right_edge) eh[k] = src[(r0 + k) * N + j0 + 2];
+    }
+  }
+
+  const bool w_ok = in_grid && j0 >= 1;                  // column j0 is interior
+  const bool e_ok = in_grid && j0 + 1 <= N - 2;          // column j0+1 is interior
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const v2d c = v[k + 1];
+    // West of j0 is the previous lane's .y; east of j0+1 is the next lane's .x.
+    double w = __shfl_up(c.y, 1, kWave);
+    double e = __shfl_down(c.x, 1, kWave);
+    if (lane == 0) w = wh[k];
+    if (lane == kWave - 1) e = eh[k];
+    if (k < rows) {
+      const int64_t row = r0 + k;
+      const double o0 = point(c.x, w, c.y, v[k + 2].x, v[k].x);
+      const double o1 = point(c.y, c.x, e, v[k + 2].y, v[k].y);
+      double* out = dst + row * N + j0;
+      if (w_ok && e_ok) {
+        *reinterpret_cast<v2d*>(out) = v2d{o0, o1};
+      } else {
+        if (w_ok) out[0] = o0;
+        if (e_ok) out[1] = o1;
+      }
+    }
+  }
+}
+
+// General fallback (odd N or unaligned grid): one point per work-item.
+__global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                                                            int64_t N) {
+  const int64_t inner = N - 2;
+  const int64_t total = inner * inner;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+    const int64_t i = 1 + t / inner, j = 1 + t % inner;
+    dst[j + i * N] = point(src[j + i * N], src[j - 1 + i * N], src[j + 1 + i * N], src[j + (i + 1) * N],
+                           src[j + (i - 1) * N]);
+  }
+}
+
+constexpr int kDefaultRows = 16;
+
+template <int R>
+int launch_vec(double* dst, const double* src, int64_t N, hipStream_t s) {
+  const int64_t strips = ceil_div(N, kCols);
+  const int64_t chunks = ceil_div(N - 2, R);
+  const int64_t grid = strips * chunks;
+  if (grid > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  jacobi_vec_kernel<R><<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(dst, src, N, int(strips), int(chunks));
+  return launch_status();
+}
+
+int sweep(double* dst, const double* src, int64_t N, const rpx_tuning* tune, hipStream_t s) {
+  if (N < 3) return RPX_SUCCESS;  // no interior
+  const bool vec = (N % 2 == 0) && aligned16(dst) && aligned16(src);
+  if (!vec) {
+    int64_t grid = ceil_div((N - 2) * (N - 2), 256);
+    if (grid > 65536) grid = 65536;
+    jacobi_scalar_kernel<<<dim3(unsigned(grid)), dim3(256), 0, s>>>(dst, src, N);
+    return launch_status();
+  }
+  switch (tune_or(tune, &rpx_tuning::items_per_thread, kDefaultRows)) {
+    case 4: return launch_vec<4>(dst, src, N, s);
+    case 8: return launch_vec<8>(dst, src, N, s);
+    case 16: return launch_vec<16>(dst, src, N, s);
+    case 32: return launch_vec<32>(dst, src, N, s);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N, const rpx_tuning* tune, void* stream) {
+  if (N < 0 || (N > 0 && (!dst || !src)) || dst == src) return RPX_ERR_INVALID_ARGUMENT;
+  return sweep(dst, src, N, tune, as_stream(stream));
+}
+
+int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps, const rpx_tuning* tune, void* stream) {
+  if (N < 0 || tsteps < 0 || (N > 0 && (!A || !B)) || A == B) return RPX_ERR_INVALID_ARGUMENT;
+  hipStream_t s = as_stream(stream);
+  for (int64_t t = 0; t < tsteps; ++t) {
+    int rc = sweep(B, A, N, tune, s);
+    if (rc != RPX_SUCCESS) return rc;
+    rc = sweep(A, B, N, tune, s);
+    if (rc != RPX_SUCCESS) return rc;
+  }
+  return RPX_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// Runtime support entry points: device selection, memory, streams, events
+// and the device-side input generators.
+//
+// Counterparts of the suite's HIP allocation/copy helpers
+// (src/common/HipDataUtils.hpp:141-255) and of DataUtils' initialisers
+// (src/common/DataUtils.cpp:477-595).
+#include <cstring>
+
+#include "rpx_common.hpp"
+
+using namespace rpx;
+
+namespace {
+
+// x[i] = lo + (hi-lo)*u_i, evaluated as two separately rounded operations
+// (no contraction) so the host restatement reproduces it exactly.
+__global__ __launch_bounds__(256) void fill_uniform_kernel(double* __restrict__ x, int64_t n,
+                                                           uint64_t state0, double lo, double w) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double u = seeded_unit(state0, uint64_t(i));
+    const double t = w * u;
+    x[i] = lo + t;
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_const_kernel(double* __restrict__ x, int64_t n, double v) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+// factor*(i + 1.1)/(i + 1.12345): IEEE double multiply and correctly rounded
+// divide, in the suite's association ((factor*(i+1.1)) / (i+1.12345)).
+__global__ __launch_bounds__(256) void fill_formula_kernel(double* __restrict__ x, int64_t n,
+                                                           double factor) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double di = static_cast<double>(i);
+    const double num = factor * (di + 1.1);
+    x[i] = num / (di + 1.12345);
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_grid_kernel(double* __restrict__ x, int64_t N, uint64_t state0,
+                                                        double lo, double w, double boundary) {
+  const int64_t n = N * N;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int64_t i = k / N, j = k - i * N;
+    if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
+      x[k] = boundary;
+    } else {
+      const double u = seeded_unit(state0, uint64_t(k));
+      const double t = w * u;
+      x[k] = lo + t;
+    }
+  }
+}
+
+// [e][s][q3] tensor field; component s of 6 (11,12,13,22,23,33) or 9
+// (row-major 3x3): diagonal in [0.5,2], off-diagonal in [-0.25,0.25].
+__global__ __launch_bounds__(256) void fill_sym_tensor_kernel(double* __restrict__ x, int64_t n, int q3, int sym,
+                                                              uint64_t state0) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double u = seeded_unit(state0, uint64_t(i));
+    const int s = int((i / q3) % sym);
+    const bool diag = (sym == 6) ? (s == 0 || s == 3 || s == 5) : (s == 0 || s == 4 || s == 8);
+    const double lo = diag ? 0.5 : -0.25;
+    const double w = diag ? 1.5 : 0.5;
+    const double t = w * u;
+    x[i] = lo + t;
+  }
+}
+
+unsigned fill_grid(int64_t n) {
+  int64_t g = ceil_div(n, 256);
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpx_abi_version(void) { return RPX_ABI_VERSION; }
+
+const char* rpx_status_string(int status) {
+  switch (status) {
+    case RPX_SUCCESS: return "success";
+    case RPX_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case RPX_ERR_SCRATCH_TOO_SMALL: return "scratch buffer too small";
+    case RPX_ERR_UNSUPPORTED: return "unsupported shape";
+    case RPX_ERR_NO_DEVICE: return "no gfx950 device";
+    case RPX_ERR_WAIT_EXPIRED: return "bounded device wait expired";
+    default: break;
+  }
+  if (status < 0 && status > -10000) return hipGetErrorString(static_cast<hipError_t>(-status));
+  return "unknown status";
+}
+
+int rpx_device_count(int* count_host) {
+  if (!count_host) return RPX_ERR_INVALID_ARGUMENT;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *count_host = 0;
+    return RPX_ERR_NO_DEVICE;
+  }
+  *count_host = c;
+  return RPX_SUCCESS;
+}
+
+int rpx_init(int device, char* arch_out, size_t arch_len) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return RPX_ERR_NO_DEVICE;
+  RPX_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  RPX_TRY(hipGetDeviceProperties(&prop, device));
+  if (arch_out && arch_len > 0) {
+    std::strncpy(arch_out, prop.gcnArchName, arch_len - 1);
+    arch_out[arch_len - 1] = '\0';
+  }
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return RPX_ERR_NO_DEVICE;
+  return RPX_SUCCESS;
+}
+
+int rpx_device_malloc(void** ptr_host, size_t bytes) {
+  if (!ptr_host) return RPX_ERR_INVALID_ARGUMENT;
+  *ptr_host = nullptr;
+  if (bytes == 0) return RPX_SUCCESS;
+  return hip_status(hipMalloc(ptr_host, bytes));
+}
+int rpx_device_free(void* ptr) { return ptr ? hip_status(hipFree(ptr)) : RPX_SUCCESS; }
+int rpx_host_malloc_pinned(void** ptr_host, size_t bytes) {
+  if (!ptr_host) return RPX_ERR_INVALID_ARGUMENT;
+  *ptr_host = nullptr;
+  if (bytes == 0) return RPX_SUCCESS;
+  return hip_status(hipHostMalloc(ptr_host, bytes, hipHostMallocDefault));
+}
+int rpx_host_free_pinned(void* ptr) { return ptr ? hip_status(hipHostFree(ptr)) : RPX_SUCCESS; }
+int rpx_memcpy_h2d(void* dst, const void* src_host, size_t bytes) {
+  if (bytes == 0) return RPX_SUCCESS;
+  if (!dst || !src_host) return RPX_ERR_INVALID_ARGUMENT;
+  return hip_status(hipMemcpy(dst, src_host, bytes, hipMemcpyHostToDevice));
+}
+int rpx_memcpy_d2h(void* dst_host, const void* src, size_t bytes) {
+  if (bytes == 0) return RPX_SUCCESS;
+  if (!dst_host || !src) return RPX_ERR_INVALID_ARGUMENT;
+  return hip_status(hipMemcpy(dst_host, src, bytes, hipMemcpyDeviceToHost));
+}
+int rpx_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return RPX_SUCCESS;
+  if (!dst || !src) return RPX_ERR_INVALID_ARGUMENT;
+  return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+}
+int rpx_stream_create(void** stream_host) {
+  if (!stream_host) return RPX_ERR_INVALID_ARGUMENT;
+  hipStream_t s = nullptr;
+  RPX_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream_host = s;
+  return RPX_SUCCESS;
+}
+int rpx_stream_destroy(void* stream) { return stream ? hip_status(hipStreamDestroy(as_stream(stream))) : RPX_SUCCESS; }
+int rpx_stream_synchronize(void* stream) { return hip_status(hipStreamSynchronize(as_stream(stream))); }
+int rpx_device_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
+int rpx_event_create(void** event_host) {
+  if (!event_host) return RPX_ERR_INVALID_ARGUMENT;
+  hipEvent_t e = nullptr;
+  RPX_TRY(hipEventCreate(&e));
+  *event_host = e;
+  return RPX_SUCCESS;
+}
+int rpx_event_destroy(void* event) {
+  return event ? hip_status(hipEventDestroy(reinterpret_cast<hipEvent_t>(event))) : RPX_SUCCESS;
+}
+int rpx_event_record(void* event, void* stream) {
+  if (!event) return RPX_ERR_INVALID_ARGUMENT;
+  return hip_status(hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream)));
+}
+int rpx_event_elapsed_ms(float* ms_host, void* start, void* stop) {
+  if (!ms_host || !start || !stop) return RPX_ERR_INVALID_ARGUMENT;
+  RPX_TRY(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)));
+  return hip_status(hipEventElapsedTime(ms_host, reinterpret_cast<hipEvent_t>(start),
+                                        reinterpret_cast<hipEvent_t>(stop)));
+}
+
+int rpx_fill_uniform_f64(double* x, int64_t n, uint64_t seed, uint64_t stream_id, double lo, double hi,
+                         void* stream) {
+  if (n < 0 || (n > 0 && !x)) return RPX_ERR_INVALID_ARGUMENT;
+  if (n == 0) return RPX_SUCCESS;
+  const double w = hi - lo;
+  fill_uniform_kernel<<<fill_grid(n), 256, 0, as_stream(stream)>>>(x, n, seeded_state0(seed, stream_id), lo, w);
+  return launch_status();
+}
+
+int rpx_fill_grid_f64(double* x, int64_t N, uint64_t seed, uint64_t stream_id, double lo, double hi,
+                      double boundary, void* stream) {
+  if (N < 0 || (N > 0 && !x)) return RPX_ERR_INVALID_ARGUMENT;
+  if (N == 0) return RPX_SUCCESS;
+  fill_grid_kernel<<<fill_grid(N * N), 256, 0, as_stream(stream)>>>(x, N, seeded_state0(seed, stream_id), lo,
+                                                                     hi - lo, boundary);
+  return launch_status();
+}
+
+int rpx_fill_sym_tensor_f64(double* x, int64_t ne, int q3, int sym, uint64_t seed, uint64_t stream_id,
+                            void* stream) {
+  if (ne < 0 || q3 <= 0 || (sym != 6 && sym != 9) || (ne > 0 && !x)) return RPX_ERR_INVALID_ARGUMENT;
+  const int64_t n = ne * int64_t(sym) * q3;
+  if (n == 0) return RPX_SUCCESS;
+  fill_sym_tensor_kernel<<<fill_grid(n), 256, 0, as_stream(stream)>>>(x, n, q3, sym,
+                                                                      seeded_state0(seed, stream_id));
+  return launch_status();
+}
+
+int rpx_fill_const_f64(double* x, int64_t n, double val, void* stream) {
+  if (n < 0 || (n > 0 && !x)) return RPX_ERR_INVALID_ARGUMENT;
+  if (n == 0) return RPX_SUCCESS;
+  fill_const_kernel<<<fill_grid(n), 256, 0, as_stream(stream)>>>(x, n, val);
+  return launch_status();
+}
+
+int rpx_fill_suite_formula_f64(double* x, int64_t n, double factor, void* stream) {
+  if (n < 0 || (n > 0 && !x)) return RPX_ERR_INVALID_ARGUMENT;
+  if (n == 0) return RPX_SUCCESS;
+  fill_formula_kernel<<<fill_grid(n), 256, 0, as_stream(stream)>>>(x, n, factor);
+  return launch_status();
+}
+
+}  // extern "C"

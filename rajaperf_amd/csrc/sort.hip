@@ -1,0 +1,323 @@
+// SORT: segmented ascending sort of FP64 keys, in place.
+//
+// Reference: src/algorithm/SORT.hpp:18-25 (rep irep sorts the length-N
+// segment x + irep*N), SORT-Seq.cpp:31-42 (std::sort), SORT-Hip.cpp:35-43
+// (RAJA::sort on the device; the suite has no hand-written GPU sort).
+//
+// MI355X design: least-significant-digit radix sort over 8 passes of 8-bit
+// digits of an order-preserving 64-bit image of the double (sign bit set for
+// non-negative values, all bits flipped for negative ones), so unsigned order
+// is numeric order with -0.0 before +0.0.  Every segment is cut into tiles of
+// 4096 keys; each pass is three launches over all tiles of all segments:
+//   hist:    per-tile digit counts (LDS atomics) -> counts[seg][digit][tile]
+//   offsets: one workgroup per segment scans its counts digit-major, giving
+//            every (digit, tile) its output offset inside the segment
+//   scatter: per tile, a stable rank of each key among equal digits (64-lane
+//            ballot matching row by row within a wavefront, then a prefix
+//            over the wavefronts), and the key is written to
+//            offset + rank.
+// A wavefront owns 1024 consecutive keys of its tile and reads them as 16
+// coalesced rows of 64, which keeps the ranking order equal to the tile order
+// (LSD stability).  Pass 0 converts doubles to keys on load; pass 7 converts
+// back on store.  The passes ping-pong between x and a scratch copy.
+#include <vector>
+
+#include "rpx_common.hpp"
+
+using namespace rpx;
+
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kRows = 16;
+constexpr int kTile = kBS * kRows;  // 4096 keys
+constexpr int kRadix = 256;
+constexpr int kPasses = 8;
+
+typedef unsigned long long u64;
+
+struct TileInfo {
+  long long seg_x;       // first key of the tile's segment in x
+  long long seg_alt;     // first key of the segment in the packed scratch copy
+  long long count_base;  // first counter of the segment
+  int len;               // keys in the tile
+  int local;             // tile index inside the segment
+  int ntiles;            // tiles of the segment
+  int seg_len;           // keys in the segment
+};
+
+struct SegInfo {
+  long long count_base;
+  int ntiles;
+  int pad;
+};
+
+// Uniform segmentation (equal lengths), or an explicit table in scratch.
+struct Plan {
+  const TileInfo* tiles;  // null => uniform
+  const SegInfo* segs;
+  long long seg_len;
+  int tiles_per_seg;
+};
+
+__device__ __forceinline__ TileInfo get_tile(const Plan& p, int t) {
+  if (p.tiles) return p.tiles[t];
+  TileInfo ti;
+  const int seg = t / p.tiles_per_seg;
+  ti.local = t % p.tiles_per_seg;
+  ti.ntiles = p.tiles_per_seg;
+  ti.seg_x = (long long)seg * p.seg_len;
+  ti.seg_alt = ti.seg_x;
+  const long long rem = p.seg_len - (long long)ti.local * kTile;
+  ti.len = int(rem < kTile ? rem : kTile);
+  ti.count_base = (long long)seg * p.tiles_per_seg * kRadix;
+  ti.seg_len = int(p.seg_len);
+  return ti;
+}
+
+__device__ __forceinline__ u64 to_key(u64 b) { return (b >> 63) ? ~b : (b | 0x8000000000000000ull); }
+__device__ __forceinline__ u64 from_key(u64 u) { return (u >> 63) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u; }
+
+__device__ __forceinline__ int tile_pos(int wave, int r, int lane) { return wave * (kRows * kWave) + r * kWave + lane; }
+
+__global__ __launch_bounds__(kBS) void hist_kernel(const u64* __restrict__ src, unsigned* __restrict__ counts, Plan plan,
+                                                   int shift, int convert, int src_is_x) {
+  __shared__ unsigned h[kRadix];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const TileInfo ti = get_tile(plan, blockIdx.x);
+  const long long sbase = (src_is_x ? ti.seg_x : ti.seg_alt) + (long long)ti.local * kTile;
+  h[tid] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int p = tile_pos(wave, r, lane);
+    if (p < ti.len) {
+      u64 k = src[sbase + p];
+      if (convert) k = to_key(k);
+      atomicAdd(&h[(k >> shift) & 0xFF], 1u);
+    }
+  }
+  __syncthreads();
+  counts[ti.count_base + (long long)tid * ti.ntiles + ti.local] = h[tid];
+}
+
+// One workgroup per segment: exclusive scan of counts in digit-major order.
+__global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ counts, Plan plan) {
+  __shared__ unsigned wsum[kBS / kWave];
+  long long base;
+  int nt;
+  if (plan.segs) {
+    base = plan.segs[blockIdx.x].count_base;
+    nt = plan.segs[blockIdx.x].ntiles;
+  } else {
+    base = (long long)blockIdx.x * plan.tiles_per_seg * kRadix;
+    nt = plan.tiles_per_seg;
+  }
+  const int d = threadIdx.x, lane = d & 63, wave = d >> 6;
+  unsigned* row = counts + base + (long long)d * nt;
+  unsigned s = 0;
+  for (int t = 0; t < nt; ++t) s += row[t];
+  // exclusive scan of the 256 row sums
+  unsigned incl = s;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const unsigned o = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += o;
+  }
+  if (lane == kWave - 1) wsum[wave] = incl;
+  __syncthreads();
+  unsigned run = incl - s;
+  for (int w = 0; w < wave; ++w) run += wsum[w];
+  for (int t = 0; t < nt; ++t) {
+    const unsigned c = row[t];
+    row[t] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(kBS) void scatter_kernel(const u64* __restrict__ src, u64* __restrict__ dst,
+                                                      const unsigned* __restrict__ counts, Plan plan, int shift,
+                                                      int convert_in, int convert_out, int src_is_x) {
+  __shared__ unsigned cnt[kBS / kWave][kRadix];
+  __shared__ unsigned goff[kRadix];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const TileInfo ti = get_tile(plan, blockIdx.x);
+  const long long sbase = (src_is_x ? ti.seg_x : ti.seg_alt) + (long long)ti.local * kTile;
+  const long long dbase = src_is_x ? ti.seg_alt : ti.seg_x;
+#pragma unroll
+  for (int w = 0; w < kBS / kWave; ++w) cnt[w][tid] = 0u;
+  goff[tid] = counts[ti.count_base + (long long)tid * ti.ntiles + ti.local];
+  __syncthreads();
+
+  u64 key[kRows];
+  unsigned rank[kRows];
+  const u64 lane_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int p = tile_pos(wave, r, lane);
+    const bool valid = p < ti.len;
+    u64 k = 0;
+    if (valid) {
+      k = src[sbase + p];
+      if (convert_in) k = to_key(k);
+    }
+    key[r] = k;
+    const unsigned d = unsigned(k >> shift) & 0xFFu;
+    u64 m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const u64 bb = __ballot((d >> b) & 1u);
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    unsigned base = 0;
+    if (valid) base = cnt[wave][d];
+    rank[r] = base + unsigned(__popcll(m & lane_below));
+    // highest lane of the group advances the wavefront's digit counter
+    if (valid && (m >> lane) == 1ull) cnt[wave][d] = base + unsigned(__popcll(m));
+  }
+  __syncthreads();
+  // Prefix over wavefronts for every digit (thread = digit).
+  {
+    unsigned run = 0;
+#pragma unroll
+    for (int w = 0; w < kBS / kWave; ++w) {
+      const unsigned c = cnt[w][tid];
+      cnt[w][tid] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int p = tile_pos(wave, r, lane);
+    if (p < ti.len) {
+      const u64 k = key[r];
+      const unsigned d = unsigned(k >> shift) & 0xFFu;
+      const long long pos = (long long)goff[d] + cnt[wave][d] + rank[r];
+      if (pos < ti.seg_len) dst[dbase + pos] = convert_out ? from_key(k) : k;
+    }
+  }
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Sizes {
+  long long total_keys = 0, total_tiles = 0;
+  bool uniform = true;
+  long long seg_len = 0;
+};
+
+Sizes measure(const int64_t* off, int64_t nseg) {
+  Sizes s;
+  for (int64_t i = 0; i < nseg; ++i) {
+    const long long len = off[i + 1] - off[i];
+    s.total_keys += len;
+    s.total_tiles += (len + kTile - 1) / kTile;
+    if (i == 0) s.seg_len = len;
+    else if (len != s.seg_len) s.uniform = false;
+  }
+  if (nseg > 0 && off[0] != 0) s.uniform = false;
+  if (s.uniform && s.seg_len == 0) s.uniform = false;
+  return s;
+}
+
+size_t scratch_for(const Sizes& s, int64_t nseg) {
+  size_t b = align256(size_t(s.total_keys) * 8);                      // ping-pong keys
+  b += align256(size_t(s.total_tiles) * kRadix * sizeof(unsigned));  // counters
+  if (!s.uniform) {
+    b += align256(size_t(s.total_tiles) * sizeof(TileInfo));
+    b += align256(size_t(nseg) * sizeof(SegInfo));
+  }
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t rpx_sort_segmented_scratch_bytes(const int64_t* offsets_host, int64_t nseg) {
+  if (!offsets_host || nseg < 0) return 0;
+  return scratch_for(measure(offsets_host, nseg), nseg);
+}
+
+int rpx_sort_segmented_f64(double* x, const int64_t* offsets_host, int64_t nseg, void* scratch, size_t scratch_bytes,
+                           void* stream) {
+  if (!offsets_host || nseg < 0) return RPX_ERR_INVALID_ARGUMENT;
+  for (int64_t i = 0; i < nseg; ++i) {
+    const long long len = offsets_host[i + 1] - offsets_host[i];
+    if (offsets_host[i] < 0 || len < 0 || len > INT32_MAX) return RPX_ERR_INVALID_ARGUMENT;
+  }
+  const Sizes s = measure(offsets_host, nseg);
+  if (s.total_keys == 0) return RPX_SUCCESS;
+  if (!x || !scratch) return RPX_ERR_INVALID_ARGUMENT;
+  if (scratch_bytes < scratch_for(s, nseg)) return RPX_ERR_SCRATCH_TOO_SMALL;
+  if (s.total_tiles > INT32_MAX || nseg > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  hipStream_t st = as_stream(stream);
+
+  // Segments with no keys contribute no tiles; non-uniform layouts (and
+  // segments not starting at x[0]) use an explicit tile table.
+  char* base = reinterpret_cast<char*>(scratch);
+  u64* alt = reinterpret_cast<u64*>(base);
+  size_t off = align256(size_t(s.total_keys) * 8);
+  unsigned* counts = reinterpret_cast<unsigned*>(base + off);
+  off += align256(size_t(s.total_tiles) * kRadix * sizeof(unsigned));
+
+  Plan plan{nullptr, nullptr, s.seg_len, int((s.seg_len + kTile - 1) / kTile)};
+  int64_t nseg_launch = nseg;
+  if (!s.uniform) {
+    // Ragged layout: explicit tile and segment tables.  Segments keep their
+    // positions in x; in the scratch copy they are packed back to back.
+    std::vector<TileInfo> tiles;
+    std::vector<SegInfo> segs;
+    tiles.reserve(size_t(s.total_tiles));
+    long long cbase = 0, packed = 0;
+    for (int64_t i = 0; i < nseg; ++i) {
+      const long long len = offsets_host[i + 1] - offsets_host[i];
+      const int nt = int((len + kTile - 1) / kTile);
+      if (nt == 0) continue;
+      for (int t = 0; t < nt; ++t) {
+        TileInfo ti;
+        ti.seg_x = offsets_host[i];
+        ti.seg_alt = packed;
+        ti.count_base = cbase;
+        ti.local = t;
+        ti.ntiles = nt;
+        const long long rem = len - (long long)t * kTile;
+        ti.len = int(rem < kTile ? rem : kTile);
+        ti.seg_len = int(len);
+        tiles.push_back(ti);
+      }
+      segs.push_back(SegInfo{cbase, nt, 0});
+      cbase += (long long)nt * kRadix;
+      packed += len;
+    }
+    TileInfo* dt = reinterpret_cast<TileInfo*>(base + off);
+    off += align256(size_t(s.total_tiles) * sizeof(TileInfo));
+    SegInfo* ds = reinterpret_cast<SegInfo*>(base + off);
+    RPX_TRY(hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(TileInfo), hipMemcpyHostToDevice, st));
+    RPX_TRY(hipMemcpyAsync(ds, segs.data(), segs.size() * sizeof(SegInfo), hipMemcpyHostToDevice, st));
+    // The tables are staged from pageable memory owned by this call: wait
+    // for the copies before it goes out of scope.
+    RPX_TRY(hipStreamSynchronize(st));
+    plan.tiles = dt;
+    plan.segs = ds;
+    nseg_launch = int64_t(segs.size());
+  }
+
+  u64* xs = reinterpret_cast<u64*>(x);
+  for (int pass = 0; pass < kPasses; ++pass) {
+    const u64* src = (pass % 2 == 0) ? xs : alt;
+    u64* dst = (pass % 2 == 0) ? alt : xs;
+    const int shift = 8 * pass;
+    const int src_is_x = (pass % 2 == 0);
+    hist_kernel<<<dim3(unsigned(s.total_tiles)), dim3(kBS), 0, st>>>(src, counts, plan, shift, pass == 0, src_is_x);
+    offsets_kernel<<<dim3(unsigned(nseg_launch)), dim3(kBS), 0, st>>>(counts, plan);
+    scatter_kernel<<<dim3(unsigned(s.total_tiles)), dim3(kBS), 0, st>>>(src, dst, counts, plan, shift, pass == 0,
+                                                                        pass == kPasses - 1, src_is_x);
+    const int rc = launch_status();
+    if (rc != RPX_SUCCESS) return rc;
+  }
+  return RPX_SUCCESS;
+}
+
+}  // extern "C"

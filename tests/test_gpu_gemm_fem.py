@@ -1,0 +1,111 @@
+"""GPU parity: GEMM (FP64 MFMA and the plain comparator) within a
+forward-error bound, MASS3DPA and DIFFUSION3DPA bit-exact against the oracle
+for every supported (D1D, Q1D), with the suite's all-ones inputs, random
+inputs (including bases that break the half-storage symmetry, where only the
+Base_Seq write order defines the result) and the seeded GLL bases."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+def dev(rpx, a):
+    return rpx.DeviceBuffer.from_array(np.ascontiguousarray(a, dtype=np.float64))
+
+
+GEMM_SHAPES = [(1, 1, 1), (17, 13, 19), (128, 128, 16), (129, 130, 33), (256, 384, 200),
+               (1000, 1000, 1200), (64, 64, 0)]
+
+
+@pytest.mark.parametrize("ni,nj,nk", GEMM_SHAPES)
+@pytest.mark.parametrize("naive", [False, True])
+def test_gemm_f64(rpx, orc, rng, ni, nj, nk, naive):
+    A = rng.uniform(-1, 1, ni * nk)
+    B = rng.uniform(-1, 1, nk * nj)
+    alpha = 1.5
+    dA, dB = dev(rpx, A), dev(rpx, B)
+    dC = dev(rpx, rng.uniform(-1, 1, ni * nj))  # overwritten: beta never reaches C
+    t = rpx.tuning(flags=rpx.RPX_TUNE_NAIVE) if naive else None
+    rpx.lib().rpx_gemm_f64(dC.ptr, dA.ptr, dB.ptr, alpha, 1.2, ni, nj, nk, t, None)
+    got = dC.to_array().reshape(ni, nj)
+    ref = orc.gemm(A, B, alpha, 1.2, ni, nj, nk).reshape(ni, nj)
+    scale = abs(alpha) * (np.abs(A.reshape(ni, nk)) @ np.abs(B.reshape(nk, nj)))
+    assert np.all(np.abs(got - ref) <= 2 * (nk + 2) * EPS * scale + 1e-300)
+
+
+def test_gemm_identity_exact(rpx, rng):
+    n = 96
+    A = rng.uniform(-1, 1, n * n)
+    dA, dI, dC = dev(rpx, A), dev(rpx, np.eye(n).reshape(-1)), rpx.DeviceBuffer(8 * n * n)
+    rpx.lib().rpx_gemm_f64(dC.ptr, dA.ptr, dI.ptr, 1.0, 0.0, n, n, n, None, None)
+    assert np.array_equal(dC.to_array(), A)
+
+
+MASS_ORDERS = [(d, q) for d in range(1, 9) for q in (d, d + 1, d + 2) if q <= 9]
+
+
+@pytest.mark.parametrize("d1d,q1d", MASS_ORDERS)
+def test_mass3dpa_bit_exact(rpx, orc, rng, d1d, q1d):
+    NE = 67
+    B = rng.uniform(-1, 1, q1d * d1d)
+    Bt = rng.uniform(-1, 1, q1d * d1d)
+    D = rng.uniform(0.5, 2, NE * q1d ** 3)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    Y0 = rng.uniform(-1, 1, NE * d1d ** 3)
+    bufs = [dev(rpx, a) for a in (B, Bt, D, X, Y0)]
+    for _ in range(2):  # Y accumulates over reps like the suite's Y +=
+        rpx.lib().rpx_mass3dpa_f64(*[b.ptr for b in bufs], NE, d1d, q1d, None, None)
+    ref = orc.mass3dpa(B, Bt, D, X, orc.mass3dpa(B, Bt, D, X, Y0, NE, d1d, q1d), NE, d1d, q1d)
+    assert np.array_equal(bufs[4].to_array(), ref)
+
+
+def test_mass3dpa_all_ones_reference_inputs(rpx):
+    NE, d1d, q1d = 1000, 4, 5
+    ones = lambda n: np.ones(n)
+    bufs = [dev(rpx, a) for a in (ones(20), ones(20), ones(NE * 125), ones(NE * 64), np.zeros(NE * 64))]
+    rpx.lib().rpx_mass3dpa_f64(*[b.ptr for b in bufs], NE, d1d, q1d, None, None)
+    assert np.all(bufs[4].to_array() == 8000.0)  # D1D^3 * Q1D^3
+
+
+@pytest.mark.parametrize("d1d", range(1, 9))
+@pytest.mark.parametrize("kind", ["random", "ones", "gll"])
+def test_diffusion3dpa_bit_exact(rpx, orc, rng, d1d, kind):
+    q1d = d1d + 1
+    NE = 23
+    if kind == "random":
+        b, g = rng.uniform(-1, 1, q1d * d1d), rng.uniform(-1, 1, q1d * d1d)
+        D = rng.uniform(-1, 1, NE * 6 * q1d ** 3)
+    elif kind == "ones":
+        b, g = np.ones(q1d * d1d), np.ones(q1d * d1d)
+        D = np.ones(NE * 6 * q1d ** 3)
+    else:
+        b, g = orc.fem_basis(d1d, q1d)
+        D = orc.fill_sym_tensor(NE, q1d ** 3, 6, 42, 2)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    Y0 = rng.uniform(-1, 1, NE * d1d ** 3)
+    bufs = [dev(rpx, a) for a in (b, g, D, X, Y0)]
+    rpx.lib().rpx_diffusion3dpa_f64(*[x.ptr for x in bufs], NE, d1d, q1d, 1, None, None)
+    assert np.array_equal(bufs[4].to_array(), orc.diffusion3dpa(b, g, D, X, Y0, NE, d1d, q1d))
+
+
+def test_diffusion3dpa_full_tensor(rpx, orc, rng):
+    d1d, q1d, NE = 3, 4, 11
+    b, g = orc.fem_basis(d1d, q1d)
+    D = rng.uniform(-1, 1, NE * 9 * q1d ** 3)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    bufs = [dev(rpx, a) for a in (b, g, D, X, np.zeros(NE * d1d ** 3))]
+    rpx.lib().rpx_diffusion3dpa_f64(*[x.ptr for x in bufs], NE, d1d, q1d, 0, None, None)
+    ref = orc.diffusion3dpa(b, g, D, X, np.zeros(NE * d1d ** 3), NE, d1d, q1d, symmetric=False)
+    assert np.array_equal(bufs[4].to_array(), ref)
+
+
+def test_fem_rejects_unsupported_orders(rpx):
+    from rajaperf_amd.abi import RpxError
+    bufs = [rpx.DeviceBuffer(8 * 1000) for _ in range(5)]
+    with pytest.raises(RpxError):
+        rpx.lib().rpx_mass3dpa_f64(*[b.ptr for b in bufs], 1, 9, 10, None, None)
+    with pytest.raises(RpxError):
+        rpx.lib().rpx_diffusion3dpa_f64(*[b.ptr for b in bufs], 1, 3, 5, 1, None, None)
