@@ -269,7 +269,7 @@ long double run_jacobi2d(const Ctx& c, int vid, orc_result* r, Timer& t) {
 
 long double run_mass3dpa(const Ctx& c, int vid, orc_result* r, Timer& t) {
   const int D1 = c.p->d1d > 0 ? c.p->d1d : 4;  // MASS3DPA.hpp:172-173
-  const int Q1 = c.p->q1d > 0 ? c.p->q1d : 5;
+  const int Q1 = c.p->q1d > 0 ? c.p->q1d : (c.p->d1d > 0 ? D1 + 1 : 5);
   const int64_t Q3 = int64_t(Q1) * Q1 * Q1, D3 = int64_t(D1) * D1 * D1;
   const int64_t target = c.target(8000 * Q3);  // MASS3DPA.cpp:26-49
   int64_t NE = (target + Q3 / 2) / Q3;
@@ -307,7 +307,7 @@ long double run_mass3dpa(const Ctx& c, int vid, orc_result* r, Timer& t) {
 
 long double run_diffusion3dpa(const Ctx& c, int vid, orc_result* r, Timer& t) {
   const int D1 = c.p->d1d > 0 ? c.p->d1d : 3;  // DIFFUSION3DPA.hpp:231-233
-  const int Q1 = c.p->q1d > 0 ? c.p->q1d : 4;
+  const int Q1 = c.p->q1d > 0 ? c.p->q1d : D1 + 1;
   const int SYM = 6;
   const int64_t Q3 = int64_t(Q1) * Q1 * Q1, D3 = int64_t(D1) * D1 * D1;
   const int64_t target = c.target(15625 * Q3);  // DIFFUSION3DPA.cpp:26-53

@@ -1,0 +1,328 @@
+// Mixed-precision GEMM with FP64 accuracy from int8 matrix cores.
+//
+// Same result semantics as the suite's GEMM (src/polybench/POLYBENCH_GEMM.hpp
+// :12-50: C = alpha*A*B, beta never reaches C).  The products run on
+// v_mfma_i32_16x16x64_i8 over error-free fixed-point slices of A and B
+// (the Ozaki splitting):
+//   * every row i of A is scaled by 2^-e_i (e_i from frexp of the row's max
+//     |A|), every column j of B by 2^-f_j, so all entries lie in (-1, 1);
+//   * each scaled entry x is cut into s signed 7-bit digits,
+//       x = sum_{p<s} d_p 2^{-7(p+1)} + r,  |d_p| <= 127,  |r| < 2^{-7s},
+//     by exact FP64 operations (scale by 128, truncate, subtract);
+//   * S_t = sum_{p+q=t} sum_k dA_p[i][k] dB_q[k][j] is an exact int32 sum for
+//     every t < s (|S_t| <= 7*127^2*nk < 2^31 for nk <= 19000);
+//   * C[i][j] = alpha * 2^(e_i+f_j) * sum_{t<s} S_t 2^{-7(t+2)}, summed in
+//     FP64 from the smallest term up.
+// The dropped digit products are below (s+3) 2^{-7s} relative to
+// 2^(e_i+f_j) per k term; with the default s = 7 the error is below 1e-12
+// relative to |alpha|(|A||B|)_ij.  There are s(s+1)/2 slice products.
+//
+// Scratch holds the int8 slices (A: [p][i][k], B: [q][j][k], K contiguous,
+// zero-padded to tile multiples) and the per-row/column exponents.
+#include "rpx_common.hpp"
+
+using namespace rpx;
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxSlices = 8;
+constexpr int kTM = 128, kTN = 128, kTK = 64;  // block tile, K step (bytes)
+constexpr int kPadRow = kTK + 16;              // LDS row stride in bytes
+constexpr int64_t kMaxK = 19000;               // keeps every S_t inside int32
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct MixedLayout {
+  int64_t nip, njp, nkp;
+  size_t a_off, b_off, ea_off, fb_off, total;
+};
+MixedLayout mixed_layout(int64_t ni, int64_t nj, int64_t nk, int s) {
+  MixedLayout l;
+  l.nip = round_up(ni, kTM);
+  l.njp = round_up(nj, kTN);
+  l.nkp = round_up(nk > 0 ? nk : 1, kTK);
+  l.a_off = 0;
+  l.b_off = align256(size_t(s) * l.nip * l.nkp);
+  l.ea_off = l.b_off + align256(size_t(s) * l.njp * l.nkp);
+  l.fb_off = l.ea_off + align256(size_t(l.nip) * 4);
+  l.total = l.fb_off + align256(size_t(l.njp) * 4);
+  return l;
+}
+
+__device__ __forceinline__ int exponent_of(double m) {
+  int e = 0;
+  if (m != 0.0) frexp(m, &e);
+  return e;
+}
+
+// Cuts x (|x| < 1) into s digits; digits[p] in [-127, 127].
+template <int S>
+__device__ __forceinline__ void cut(double x, int* digits) {
+#pragma unroll
+  for (int p = 0; p < S; ++p) {
+    const double t = x * 128.0;
+    const double d = trunc(t);
+    digits[p] = int(d);
+    x = t - d;
+  }
+}
+
+// One workgroup per (padded) row of A: row exponent, then the slices.
+template <int S>
+__global__ __launch_bounds__(256) void split_rows_kernel(const double* __restrict__ A, int64_t ni, int64_t nk,
+                                                         int64_t nkp, int64_t nip, signed char* __restrict__ As,
+                                                         int* __restrict__ ea) {
+  __shared__ double red[4];
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x;
+  double m = 0.0;
+  if (i < ni)
+    for (int64_t k = tid; k < nk; k += 256) m = fmax(m, fabs(A[i * nk + k]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const int e = exponent_of(m);
+  if (tid == 0) ea[i] = e;
+  // 16 consecutive k per work-item per step; one 16-byte store per slice.
+  for (int64_t k0 = int64_t(tid) * 16; k0 < nkp; k0 += 256 * 16) {
+    int dig[16][S];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int64_t k = k0 + u;
+      const double x = (i < ni && k < nk) ? ldexp(A[i * nk + k], -e) : 0.0;
+      cut<S>(x, dig[u]);
+    }
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      v4i w;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        w[c] = (dig[4 * c][p] & 0xFF) | ((dig[4 * c + 1][p] & 0xFF) << 8) | ((dig[4 * c + 2][p] & 0xFF) << 16) |
+               ((dig[4 * c + 3][p] & 0xFF) << 24);
+      *reinterpret_cast<v4i*>(As + (int64_t(p) * nip + i) * nkp + k0) = w;
+    }
+  }
+}
+
+// Column maxima of B as exponents: atomicMax over K chunks.
+__global__ __launch_bounds__(256) void col_exponent_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
+                                                           int64_t kchunk, int* __restrict__ fb) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= nj) return;
+  const int64_t kb = int64_t(blockIdx.y) * kchunk;
+  const int64_t ke = (kb + kchunk < nk) ? kb + kchunk : nk;
+  double m = 0.0;
+  for (int64_t k = kb; k < ke; ++k) m = fmax(m, fabs(B[k * nj + j]));
+  if (m != 0.0) atomicMax(&fb[j], exponent_of(m));
+}
+
+__global__ void fill_int_kernel(int* p, int64_t n, int v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// B -> Bs[q][j][k] through an LDS transpose: a workgroup owns 64 columns x 64 k.
+template <int S>
+__global__ __launch_bounds__(256) void split_cols_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
+                                                         int64_t nkp, int64_t njp, const int* __restrict__ fb,
+                                                         signed char* __restrict__ Bs) {
+  __shared__ __attribute__((aligned(16))) signed char lds[S][64][64 + 16];
+  const int64_t j0 = int64_t(blockIdx.x) * 64, k0 = int64_t(blockIdx.y) * 64;
+  const int tid = threadIdx.x;
+  const int jj = tid & 63;
+  const int64_t j = j0 + jj;
+  const int f = (j < nj) ? fb[j] : 0;
+  for (int kk = tid >> 6; kk < 64; kk += 4) {
+    const int64_t k = k0 + kk;
+    const double x = (j < nj && k < nk) ? ldexp(B[k * nj + j], -(f == INT32_MIN ? 0 : f)) : 0.0;
+    int dig[S];
+    cut<S>(x, dig);
+#pragma unroll
+    for (int q = 0; q < S; ++q) lds[q][jj][kk] = (signed char)dig[q];
+  }
+  __syncthreads();
+  // 64 columns x 64 bytes per slice: 16 bytes per work-item.
+  const int row = tid >> 2, part = (tid & 3) * 16;
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const v4i w = *reinterpret_cast<const v4i*>(&lds[q][row][part]);
+    *reinterpret_cast<v4i*>(Bs + (int64_t(q) * njp + j0 + row) * nkp + k0 + part) = w;
+  }
+}
+
+__device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned G) {
+  return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+}
+
+// 128x128 output tile per workgroup of four wavefronts (64x64 each, a 4x4
+// grid of 16x16 int32 accumulators).  For t = s-1 down to 0 the K loop runs
+// over the concatenation of the slice pairs (p, t-p); S_t is then folded into
+// FP64 accumulators with its power-of-two weight.
+template <int S>
+__global__ __launch_bounds__(256) void ozaki_gemm_kernel(double* __restrict__ C, const signed char* __restrict__ As,
+                                                         const signed char* __restrict__ Bs,
+                                                         const int* __restrict__ ea, const int* __restrict__ fb,
+                                                         double alpha, int64_t ni, int64_t nj, int64_t nip,
+                                                         int64_t njp, int64_t nkp, int tiles_n) {
+  __shared__ __attribute__((aligned(16))) signed char Al[2][kTM * kPadRow];
+  __shared__ __attribute__((aligned(16))) signed char Bl[2][kTN * kPadRow];
+  const unsigned tile = xcd_major(blockIdx.x, gridDim.x);
+  const int64_t m0 = int64_t(tile / unsigned(tiles_n)) * kTM;
+  const int64_t n0 = int64_t(tile % unsigned(tiles_n)) * kTN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int64_t ksteps = nkp / kTK;
+
+  // Global -> register staging: two 16-byte chunks of A and of B per work-item.
+  const int c0 = tid, c1 = tid + 256;
+  const int r0 = c0 >> 2, r1 = c1 >> 2, o0 = (c0 & 3) * 16, o1 = (c1 & 3) * 16;
+
+  v4d cacc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cacc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int t = S - 1; t >= 0; --t) {
+    const int p_lo = t - (S - 1) > 0 ? t - (S - 1) : 0;
+    const int p_hi = t < S - 1 ? t : S - 1;
+    const int64_t nsteps = int64_t(p_hi - p_lo + 1) * ksteps;
+    v4i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+
+    v4i ga0, ga1, gb0, gb1;
+    auto gload = [&](int64_t st) {
+      const int p = p_lo + int(st / ksteps);
+      const int q = t - p;
+      const int64_t k0 = (st % ksteps) * kTK;
+      const signed char* a = As + (int64_t(p) * nip + m0) * nkp + k0;
+      const signed char* b = Bs + (int64_t(q) * njp + n0) * nkp + k0;
+      ga0 = *reinterpret_cast<const v4i*>(a + int64_t(r0) * nkp + o0);
+      ga1 = *reinterpret_cast<const v4i*>(a + int64_t(r1) * nkp + o1);
+      gb0 = *reinterpret_cast<const v4i*>(b + int64_t(r0) * nkp + o0);
+      gb1 = *reinterpret_cast<const v4i*>(b + int64_t(r1) * nkp + o1);
+    };
+    auto lstore = [&](int buf) {
+      *reinterpret_cast<v4i*>(&Al[buf][r0 * kPadRow + o0]) = ga0;
+      *reinterpret_cast<v4i*>(&Al[buf][r1 * kPadRow + o1]) = ga1;
+      *reinterpret_cast<v4i*>(&Bl[buf][r0 * kPadRow + o0]) = gb0;
+      *reinterpret_cast<v4i*>(&Bl[buf][r1 * kPadRow + o1]) = gb1;
+    };
+
+    gload(0);
+    __syncthreads();  // previous t's readers are done with buffer 0
+    lstore(0);
+    __syncthreads();
+    for (int64_t st = 0; st < nsteps; ++st) {
+      const int buf = int(st & 1);
+      if (st + 1 < nsteps) gload(st + 1);
+      v4i af[4], bf[4];
+      const int fr = lane & 15, fk = (lane >> 4) * 16;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const v4i*>(&Al[buf][(wm + i * 16 + fr) * kPadRow + fk]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const v4i*>(&Bl[buf][(wn + j * 16 + fr) * kPadRow + fk]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+      if (st + 1 < nsteps) {
+        lstore(buf ^ 1);
+        __syncthreads();
+      }
+    }
+    const double w = ldexp(1.0, -7 * (t + 2));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cacc[i][j][r] += double(acc[i][j][r]) * w;
+  }
+
+  // int32 16x16 accumulator layout: element r of lane l is row 4*(l/16)+r,
+  // column l%16.
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t gj = n0 + wn + j * 16 + (lane & 15);
+        if (gi < ni && gj < nj) {
+          const int fj = fb[gj];
+          const double v = ldexp(cacc[i][j][r], ea[gi] + (fj == INT32_MIN ? 0 : fj));
+          C[gi * nj + gj] = alpha * v;
+        }
+      }
+}
+
+template <int S>
+int run_mixed(double* C, const double* A, const double* B, double alpha, int64_t ni, int64_t nj, int64_t nk,
+              const MixedLayout& l, char* base, hipStream_t s) {
+  signed char* As = reinterpret_cast<signed char*>(base + l.a_off);
+  signed char* Bs = reinterpret_cast<signed char*>(base + l.b_off);
+  int* ea = reinterpret_cast<int*>(base + l.ea_off);
+  int* fb = reinterpret_cast<int*>(base + l.fb_off);
+  split_rows_kernel<S><<<dim3(unsigned(l.nip)), dim3(256), 0, s>>>(A, ni, nk, l.nkp, l.nip, As, ea);
+  fill_int_kernel<<<dim3(unsigned(ceil_div(l.njp, 256))), dim3(256), 0, s>>>(fb, l.njp, INT32_MIN);
+  const int64_t kchunk = 256;
+  if (nk > 0)
+    col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 256)), unsigned(ceil_div(nk, kchunk))), dim3(256), 0, s>>>(
+        B, nk, nj, kchunk, fb);
+  split_cols_kernel<S><<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 64)), dim3(256), 0, s>>>(B, nk, nj, l.nkp,
+                                                                                              l.njp, fb, Bs);
+  const int64_t tm = l.nip / kTM, tn = l.njp / kTN;
+  ozaki_gemm_kernel<S><<<dim3(unsigned(tm * tn)), dim3(256), 0, s>>>(C, As, Bs, ea, fb, alpha, ni, nj, l.nip, l.njp,
+                                                                    l.nkp, int(tn));
+  return launch_status();
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk, int nslices) {
+  if (ni < 0 || nj < 0 || nk < 0) return 0;
+  const int s = nslices <= 0 ? 7 : nslices;
+  if (s > kMaxSlices) return 0;
+  return mixed_layout(ni, nj, nk, s).total;
+}
+
+int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha, double /*beta*/, int64_t ni,
+                       int64_t nj, int64_t nk, int nslices, void* scratch, size_t scratch_bytes, void* stream) {
+  if (ni < 0 || nj < 0 || nk < 0 || nslices < 0 || nslices > kMaxSlices) return RPX_ERR_INVALID_ARGUMENT;
+  if (ni == 0 || nj == 0) return RPX_SUCCESS;
+  if (!C || !scratch || (nk > 0 && (!A || !B))) return RPX_ERR_INVALID_ARGUMENT;
+  if (nk > kMaxK) return RPX_ERR_UNSUPPORTED;
+  const int s = nslices == 0 ? 7 : nslices;
+  const MixedLayout l = mixed_layout(ni, nj, nk, s);
+  if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
+  if (l.nip > INT32_MAX || (l.nip / kTM) * (l.njp / kTN) > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  char* base = reinterpret_cast<char*>(scratch);
+  hipStream_t st = as_stream(stream);
+  switch (s) {
+    case 1: return run_mixed<1>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 2: return run_mixed<2>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 3: return run_mixed<3>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 4: return run_mixed<4>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 5: return run_mixed<5>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 6: return run_mixed<6>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 7: return run_mixed<7>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    case 8: return run_mixed<8>(C, A, B, alpha, ni, nj, nk, l, base, st);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+}  // extern "C"

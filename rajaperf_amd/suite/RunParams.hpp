@@ -1,0 +1,101 @@
+// Run parameters and command-line parsing (src/common/RunParams.hpp,
+// RunParams.cpp:31-90 defaults, :296-1268 parser, :1893-2490 selection).
+//
+// Kept from the suite: -k/--kernels, -ek/--exclude-kernels, -v/--variants,
+// -ev/--exclude-variants, -t/--tunings, -et/--exclude-tunings, --size,
+// --sizefact, --repfact, --npasses, --checkrun, --gpu_block_size,
+// --items_per_thread, -od/--outdir, -of/--outfile, --refvar, --dryrun,
+// --disable-warmup, --show-progress, --print-kernels, --print-variants,
+// -h/--help.  Added for this variant: --input-mode {reference,seeded},
+// --seed, --tsteps (JACOBI_2D), --nk (GEMM), --d1d/--q1d (FEM orders),
+// --json <file> (machine-readable results).
+#pragma once
+
+#include <cstdint>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "RAJAPerfSuite.hpp"
+
+namespace rajaperf {
+
+class RunParams {
+ public:
+  enum InputOpt { InfoRequest, DryRun, CheckRun, PerfRun, BadInput, Undefined };
+  enum SizeMeaning { Unset, Factor, Direct };
+  enum InputMode { ReferenceInputs, SeededInputs };
+
+  RunParams(int argc, const char** argv);
+
+  InputOpt getInputState() const { return input_state; }
+  SizeMeaning getSizeMeaning() const { return size_meaning; }
+  double getSizeFactor() const { return size_factor; }
+  std::int64_t getSize() const { return size; }
+  double getRepFactor() const { return rep_fact; }
+  int getNumPasses() const { return npasses; }
+  int getCheckRunReps() const { return checkrun_reps; }
+  bool showProgress() const { return show_progress; }
+  bool getDisableWarmup() const { return disable_warmup; }
+  const std::string& getOutputDirName() const { return outdir; }
+  const std::string& getOutputFilePrefix() const { return outfile_prefix; }
+  const std::string& getJsonFile() const { return json_file; }
+  InputMode getInputMode() const { return input_mode; }
+  std::uint64_t getSeed() const { return seed; }
+  std::int64_t getTsteps() const { return tsteps; }
+  std::int64_t getNk() const { return nk; }
+  int getD1D() const { return d1d; }
+  int getQ1D() const { return q1d; }
+  int getGPUBlockSize() const { return gpu_block_size; }
+  int getItemsPerThread() const { return items_per_thread; }
+
+  const std::set<KernelID>& getKernelIDsToRun() const { return run_kernels; }
+  const std::set<VariantID>& getVariantIDsToRun() const { return run_variants; }
+  VariantID getReferenceVariantID() const { return reference_vid; }
+  // Tuning selection: empty = all.
+  const std::vector<std::string>& getTuningInput() const { return tuning_input; }
+  const std::vector<std::string>& getExcludeTuningInput() const { return exclude_tuning_input; }
+  const std::vector<std::string>& getInvalidInput() const { return invalid_input; }
+
+  void printHelpMessage(std::ostream& str) const;
+  void print(std::ostream& str) const;
+
+ private:
+  void parse(int argc, const char** argv);
+  void processKernelInput();
+  void processVariantInput();
+
+  InputOpt input_state = Undefined;
+  SizeMeaning size_meaning = Unset;
+  double size_factor = 1.0;
+  std::int64_t size = 0;
+  double rep_fact = 1.0;
+  int npasses = 1;
+  int checkrun_reps = 1;
+  bool show_progress = false;
+  bool disable_warmup = false;
+  bool print_kernels = false;
+  bool print_variants = false;
+  std::string outdir = ".";
+  std::string outfile_prefix = "RAJAPerf";
+  std::string json_file;
+  InputMode input_mode = ReferenceInputs;
+  std::uint64_t seed = 42;
+  std::int64_t tsteps = 0;
+  std::int64_t nk = 0;
+  int d1d = 0, q1d = 0;
+  int gpu_block_size = 0;
+  int items_per_thread = 0;
+
+  std::vector<std::string> kernel_input, exclude_kernel_input;
+  std::vector<std::string> variant_input, exclude_variant_input;
+  std::vector<std::string> tuning_input, exclude_tuning_input;
+  std::vector<std::string> invalid_input;
+  std::string reference_variant;
+
+  std::set<KernelID> run_kernels;
+  std::set<VariantID> run_variants;
+  VariantID reference_vid = NumVariants;
+};
+
+}  // namespace rajaperf

@@ -1,0 +1,126 @@
+// Polybench group: constructors and setUp/updateChecksum/tearDown follow
+// src/polybench/POLYBENCH_GEMM.cpp:22-96 and POLYBENCH_JACOBI_2D.cpp:22-95.
+#include "polybench/Polybench.hpp"
+
+#include <cmath>
+
+#include "stream/Stream.hpp"
+
+namespace rajaperf {
+namespace polybench {
+
+// ---- GEMM ------------------------------------------------------------------------
+POLYBENCH_GEMM::POLYBENCH_GEMM(const RunParams& params) : KernelBase(Polybench_GEMM, params) {
+  const Index_type ni_default = 1000, nj_default = 1000, nk_default = 1200;
+  setDefaultProblemSize(ni_default * nj_default);
+  setDefaultReps(4);
+  m_ni = static_cast<Index_type>(std::sqrt(double(getTargetProblemSize())) + std::sqrt(2.0) - 1);
+  m_nj = m_ni;
+  m_nk = params.getNk() > 0 ? params.getNk() : Index_type(double(nk_default) / ni_default * m_ni);
+  // Suite constants (POLYBENCH_GEMM.cpp:35-36); the seeded workload uses 1.5 / 1.2.
+  m_alpha = params.getInputMode() == RunParams::SeededInputs ? 1.5 : 0.62;
+  m_beta = params.getInputMode() == RunParams::SeededInputs ? 1.2 : 1.002;
+  setActualProblemSize(m_ni * m_nj);
+  setItsPerRep(m_ni * m_nj);
+  setKernelsPerRep(1);
+  setBytesReadPerRep(1 * sizeof(Real_type) * m_ni * m_nk + 1 * sizeof(Real_type) * m_nj * m_nk);
+  setBytesWrittenPerRep(1 * sizeof(Real_type) * m_ni * m_nj);
+  setBytesAtomicModifyWrittenPerRep(0);
+  setFLOPsPerRep((1 + 3 * m_nk) * m_ni * m_nj);
+  checksum_scale_factor = 0.001 * (static_cast<Checksum_type>(getDefaultProblemSize()) / getActualProblemSize());
+  setComplexity(Complexity::N_to_the_three_halves);
+  setVariantDefined(MI355X_Native);
+}
+void POLYBENCH_GEMM::setMI355XTuningDefinitions(VariantID vid) {
+  addVariantTuningName(vid, "default");  // FP64 on the MFMA f64 units
+  addVariantTuningName(vid, "naive");    // one thread per C element, the plain FP64 comparator
+}
+void POLYBENCH_GEMM::setUp(VariantID, size_t) {
+  allocAndInitData(m_A, Size_type(m_ni * m_nk), -1.0, 1.0);
+  allocAndInitData(m_B, Size_type(m_nk * m_nj), -1.0, 1.0);
+  allocAndInitDataConst(m_C, Size_type(m_ni * m_nj), 0.0);
+}
+void POLYBENCH_GEMM::updateChecksum(VariantID vid, size_t t) {
+  checksum[vid][t] += calcChecksum(m_C, Size_type(m_ni * m_nj), Real_type(checksum_scale_factor));
+}
+void POLYBENCH_GEMM::tearDown(VariantID, size_t) {
+  deallocData(m_A);
+  deallocData(m_B);
+  deallocData(m_C);
+}
+void POLYBENCH_GEMM::runMI355XVariant(VariantID, size_t tune_idx) {
+  const Index_type reps = getRunReps();
+  rpx_tuning tune = defaultTuning();
+  if (tune_idx == 1) tune.flags |= RPX_TUNE_NAIVE;
+  startTimer();
+  for (RepIndex_type irep = 0; irep < reps; ++irep)
+    rpxCheck(rpx_gemm_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, &tune, nullptr), "GEMM");
+  stopTimer();
+}
+
+// ---- JACOBI_2D -------------------------------------------------------------------
+POLYBENCH_JACOBI_2D::POLYBENCH_JACOBI_2D(const RunParams& params) : KernelBase(Polybench_JACOBI_2D, params) {
+  const Index_type N_default = 1002;
+  setDefaultProblemSize((N_default - 2) * (N_default - 2));
+  setDefaultReps(50);
+  m_N = static_cast<Index_type>(std::sqrt(double(getTargetProblemSize())) + 2 + std::sqrt(2.0) - 1);
+  m_tsteps = params.getTsteps() > 0 ? params.getTsteps() : 40;
+  setActualProblemSize((m_N - 2) * (m_N - 2));
+  setItsPerRep(m_tsteps * (2 * (m_N - 2) * (m_N - 2)));
+  setKernelsPerRep(2);
+  setBytesReadPerRep((1 * sizeof(Real_type) * (m_N * m_N - 4) + 1 * sizeof(Real_type) * (m_N * m_N - 4)) *
+                     m_tsteps);
+  setBytesWrittenPerRep(
+      (1 * sizeof(Real_type) * (m_N - 2) * (m_N - 2) + 1 * sizeof(Real_type) * (m_N - 2) * (m_N - 2)) * m_tsteps);
+  setBytesAtomicModifyWrittenPerRep(0);
+  setFLOPsPerRep(m_tsteps * (5 * (m_N - 2) * (m_N - 2) + 5 * (m_N - 2) * (m_N - 2)));
+  checksum_scale_factor = 0.0001 * (static_cast<Checksum_type>(getDefaultProblemSize()) / getActualProblemSize());
+  setComplexity(Complexity::N);
+  setVariantDefined(MI355X_Native);
+}
+void POLYBENCH_JACOBI_2D::setMI355XTuningDefinitions(VariantID vid) {
+  addVariantTuningName(vid, "default");
+  addVariantTuningName(vid, "rows_8");
+  addVariantTuningName(vid, "rows_32");
+}
+void POLYBENCH_JACOBI_2D::setUp(VariantID, size_t) {
+  const Size_type nn = Size_type(m_N * m_N);
+  allocData(m_Ainit, nn);
+  allocData(m_Binit, nn);
+  if (seededInputs()) {  // interior uniform [0,1), zero boundary
+    detail::initDataSeededGridDevice(m_Ainit, m_N, run_params.getSeed());
+    detail::initDataSeededGridDevice(m_Binit, m_N, run_params.getSeed());
+  } else {
+    detail::initDataDevice(m_Ainit, nn);
+    detail::initDataDevice(m_Binit, nn);
+  }
+  allocData(m_A, nn);
+  allocData(m_B, nn);
+}
+void POLYBENCH_JACOBI_2D::updateChecksum(VariantID vid, size_t t) {
+  checksum[vid][t] += calcChecksum(m_A, Size_type(m_N * m_N), Real_type(checksum_scale_factor));
+  checksum[vid][t] += calcChecksum(m_B, Size_type(m_N * m_N), Real_type(checksum_scale_factor));
+}
+void POLYBENCH_JACOBI_2D::tearDown(VariantID, size_t) {
+  deallocData(m_A);
+  deallocData(m_B);
+  deallocData(m_Ainit);
+  deallocData(m_Binit);
+}
+void POLYBENCH_JACOBI_2D::runMI355XVariant(VariantID, size_t tune_idx) {
+  const Index_type reps = getRunReps();
+  rpx_tuning tune = defaultTuning();
+  if (tune_idx == 1) tune.items_per_thread = 8;
+  if (tune_idx == 2) tune.items_per_thread = 32;
+  // POLYBENCH_JACOBI_2D_DATA_SETUP copies the initial grids before the timer.
+  const Size_type bytes = Size_type(m_N * m_N) * sizeof(Real_type);
+  rpxCheck(rpx_memcpy_d2d(m_A, m_Ainit, bytes, nullptr), "JACOBI_2D copy");
+  rpxCheck(rpx_memcpy_d2d(m_B, m_Binit, bytes, nullptr), "JACOBI_2D copy");
+  startTimer();
+  for (RepIndex_type irep = 0; irep < reps; ++irep)
+    rpxCheck(rpx_jacobi2d_f64(m_A, m_B, m_N, m_tsteps, &tune, nullptr), "JACOBI_2D");
+  stopTimer();
+}
+
+}  // namespace polybench
+}  // namespace rajaperf

@@ -1,0 +1,92 @@
+"""CPU tests of the boundary: the C ABI library loads and exports every symbol
+the header declares (no compute calls), and the suite harness parses its
+command line, selects kernels/variants and reports bad input like the suite
+(src/common/RunParams.cpp, Executor.cpp:360-373)."""
+
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+from rajaperf_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "rajaperf_amd", "bin", "rajaperf-mi355x")
+
+
+def test_library_exports_every_declared_symbol():
+    declared = abi.declared_symbols()
+    assert len(declared) >= 40
+    dll = ctypes.CDLL(abi.KERNEL_LIB)
+    missing = [s for s in declared if not hasattr(dll, s)]
+    assert missing == []
+    # and the binding knows every declared entry point
+    assert sorted(set(declared) - set(abi._SIGS)) == []
+
+
+def test_abi_version_and_status_strings():
+    L = abi.lib()
+    assert L.rpx_abi_version() == 1
+    assert L.rpx_status_string(abi.RPX_ERR_SCRATCH_TOO_SMALL) == b"scratch buffer too small"
+    assert L.rpx_status_string(0) == b"success"
+
+
+def test_argument_errors_need_no_device():
+    L = abi.lib().raw
+    # invalid arguments are rejected before any device call
+    assert L("rpx_triad_f64")(None, None, None, 1.0, 10, None, None) == abi.RPX_ERR_INVALID_ARGUMENT
+    assert L("rpx_copy_f64")(None, None, -1, None, None) == abi.RPX_ERR_INVALID_ARGUMENT
+    assert L("rpx_diffusion3dpa_f64")(8, 8, 8, 8, 8, 1, 3, 5, 1, None, None) == abi.RPX_ERR_UNSUPPORTED
+    assert L("rpx_mass3dpa_f64")(8, 8, 8, 8, 8, 1, 9, 9, None, None) == abi.RPX_ERR_UNSUPPORTED
+    o = abi.offsets_array([5, 2])
+    assert L("rpx_sort_segmented_f64")(8, o, 1, 8, 1 << 20, None) == abi.RPX_ERR_INVALID_ARGUMENT
+    assert L("rpx_jacobi2d_f64")(8, 8, 10, 1, None, None) == abi.RPX_ERR_INVALID_ARGUMENT  # A == B
+
+
+def test_scratch_queries():
+    L = abi.lib()
+    assert L.rpx_reduce_scratch_bytes(1 << 28, None) >= 256 + 8 * 2048
+    assert L.rpx_scan_scratch_bytes(1 << 28, None) >= 3 * 8 * (1 << 16)
+    offs = abi.offsets_array([0, 100, 100, 5000])
+    assert L.rpx_sort_segmented_scratch_bytes(offs, 3) >= 8 * 5000
+
+
+def run_harness(*args):
+    return subprocess.run([HARNESS, *args], capture_output=True, text=True, timeout=120)
+
+
+def test_harness_lists_kernels_and_variant():
+    out = run_harness("--print-kernels").stdout
+    for k in ["Stream_TRIAD", "Stream_DOT", "Polybench_GEMM", "Polybench_JACOBI_2D", "Apps_MASS3DPA",
+              "Apps_DIFFUSION3DPA", "Algorithm_SCAN", "Algorithm_REDUCE_SUM", "Algorithm_SORT"]:
+        assert k in out
+    assert "MI355X_Native" in run_harness("--print-variants").stdout
+
+
+def test_harness_bad_input_runs_nothing():
+    r = run_harness("-k", "NotAKernel")
+    assert r.returncode == 1 and "Bad input" in r.stdout and "NotAKernel" in r.stdout
+    r = run_harness("-v", "Base_Seq")  # provided by the suite, not by this harness
+    assert r.returncode == 1 and "not built into this harness" in r.stdout
+    r = run_harness("--size", "10", "--sizefact", "2")  # Direct and Factor sizes conflict
+    assert r.returncode == 1
+
+
+def test_harness_dryrun_shapes_match_reference_formulas():
+    r = run_harness("--dryrun", "-k", "Polybench", "Apps", "--size", str(8190 * 8190))
+    assert r.returncode == 0
+    lines = {l.split()[0]: l.split() for l in r.stdout.splitlines() if l.strip().startswith(("Polybench", "Apps"))}
+    assert lines["Polybench_JACOBI_2D"][2] == str(8190 * 8190)       # N = 8192
+    assert lines["Polybench_GEMM"][2] == str(8190 * 8190)            # ni = nj = 8190
+    r = run_harness("--dryrun", "-k", "GEMM", "--size", str(4096 * 4096), "--nk", "4096")
+    gemm = [l for l in r.stdout.splitlines() if "Polybench_GEMM" in l][0].split()
+    assert gemm[6] == str(8 * (2 * 4096 * 4096) + 8 * 4096 * 4096)  # bytes/rep with nk = 4096
+    assert gemm[8] == str((1 + 3 * 4096) * 4096 * 4096)               # POLYBENCH_GEMM.cpp:47-48
+
+
+def test_harness_without_device_fails_cleanly():
+    if abi.device_count() > 0:
+        pytest.skip("a device is present")
+    r = run_harness("-k", "TRIAD", "--checkrun")
+    assert r.returncode == 3 and "no gfx950 device" in r.stderr

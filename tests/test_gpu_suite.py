@@ -1,0 +1,65 @@
+"""GPU: the suite harness end to end, in the pattern of the suite's own test
+(test/test-raja-perf-suite.cpp:50-178): run every kernel through
+KernelBase::execute for the MI355X_Native variant with --checkrun and
+several passes, then compare each checksum with the oracle's Base_Seq
+checksum for the same inputs.  The suite's tolerance is 1e-7 absolute; the
+element-wise, stencil, FEM and sort kernels are held to equality of the
+long-double checksum."""
+
+import pytest
+
+from rajaperf_amd.suite import run_suite
+
+pytestmark = pytest.mark.gpu
+
+BIT_EXACT = {"Stream_COPY", "Stream_MUL", "Stream_ADD", "Stream_TRIAD", "Polybench_JACOBI_2D", "Apps_MASS3DPA",
+             "Apps_DIFFUSION3DPA", "Algorithm_SORT"}
+
+CASES = [
+    # (kernel, size, extra args)
+    ("Stream_COPY", 100_003, []), ("Stream_MUL", 100_003, []), ("Stream_ADD", 100_003, []),
+    ("Stream_TRIAD", 100_003, []), ("Stream_DOT", 100_003, []),
+    ("Algorithm_REDUCE_SUM", 100_003, []), ("Algorithm_SCAN", 100_003, []), ("Algorithm_SORT", 20_011, []),
+    ("Polybench_JACOBI_2D", 130 * 130, ["--tsteps", "5"]), ("Polybench_GEMM", 100 * 100, []),
+    ("Apps_MASS3DPA", 125 * 300, []), ("Apps_DIFFUSION3DPA", 64 * 300, []),
+    ("Apps_MASS3DPA", 216 * 50, ["--d1d", "5", "--q1d", "6"]),
+    ("Apps_DIFFUSION3DPA", 125 * 50, ["--d1d", "4"]),
+]
+
+
+@pytest.mark.parametrize("mode", ["reference", "seeded"])
+@pytest.mark.parametrize("kernel,size,extra", CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(CASES)])
+def test_harness_checksum_matches_base_seq(rpx, orc, kernel, size, extra, mode):
+    args = ["-k", kernel, "--size", str(size), "--checkrun", "3", "--npasses", "2", "--input-mode", mode,
+            "-t", "default", *extra]
+    res = run_suite(args)
+    assert res["rc"] == 0, res.get("failures")
+    (run,) = [r for r in res["runs"] if r["kernel"] == kernel]
+    kw = dict(size=size, checkrun=3, npasses=2, input_mode=1 if mode == "seeded" else 0)
+    for a, b in zip(extra[::2], extra[1::2]):
+        kw[a.lstrip("-")] = int(b)
+    ref = orc.run(kernel, "Base_Seq", **kw)
+    assert run["problem_size"] == ref["problem_size"] and run["reps"] == ref["reps"]
+    assert run["bytes_per_rep"] == ref["bytes_per_rep"]
+    from fractions import Fraction
+    ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
+    diff = abs(float(run["checksum"] - ref_cs))
+    assert diff < 1e-7, (kernel, float(run["checksum"]), float(ref_cs))  # test-raja-perf-suite.cpp:167
+    if kernel in BIT_EXACT:
+        assert run["checksum"] == ref_cs
+    else:
+        assert diff <= 1e-12 * max(1.0, abs(float(ref_cs)))
+
+
+def test_harness_all_tunings_run_and_agree(rpx):
+    res = run_suite(["-k", "Stream", "Polybench_JACOBI_2D", "Algorithm_REDUCE_SUM", "Polybench_GEMM",
+                     "--size", "40000", "--checkrun", "2", "--input-mode", "seeded"])
+    assert res["rc"] == 0, res.get("failures")
+    by_kernel = {}
+    for r in res["runs"]:
+        by_kernel.setdefault(r["kernel"], []).append(r)
+    for k, runs in by_kernel.items():
+        assert len(runs) >= 2
+        cs = [float(r["checksum"]) for r in runs]
+        assert max(cs) - min(cs) < 1e-7, k
+        assert all(r["time_total_s"] > 0 for r in runs)
