@@ -1,0 +1,212 @@
+"""Per-kernel roofline report of the MI355X variant.
+
+    python -m rajaperf_amd.roofline [--configs 2,3,4,5] [--reps 5] [--csv FILE]
+
+Times each C-ABI entry point on device-resident seeded inputs with HIP
+events (after warm-up) and reports, per kernel, the time per rep and the
+throughput against its roofline: algorithmic bytes (the suite's bytes per
+rep) over time against 8 TB/s HBM for the memory-bound kernels, 2*M*N*K
+flops over time against the MFMA peak for GEMM.  Config 4 also reports the
+mixed-precision GEMM's speedup over the FP64 paths and its error, per
+element, relative to |alpha|(|A||B|)_ij on sampled rows checked in 80-bit
+arithmetic on the host.
+
+Configurations (BASELINE workloads):
+  2  TRIAD/DOT/REDUCE_SUM/SCAN (+COPY/MUL/ADD) at N=2^28; JACOBI_2D 8192^2
+  3  MASS3DPA and DIFFUSION3DPA, NE=262,144, D1D=4, Q1D=5
+  4  GEMM 4096^3 (FP64 MFMA, the plain FP64 comparator, mixed precision)
+  5  SORT and SCAN over 256 segments, lengths log-uniform in [2^10, 2^20]
+"""
+
+from __future__ import annotations
+
+import argparse
+import csv
+import sys
+
+import numpy as np
+
+from . import abi
+
+HBM_GBS = 8000.0
+FP64_TFLOPS = 78.6
+INT8_TOPS = 5000.0
+SEED = 42
+
+
+def _time(fn, reps, warmup=2):
+    for _ in range(warmup):
+        fn()
+    abi.synchronize()
+    e0, e1 = abi.Event(), abi.Event()
+    e0.record(None)
+    for _ in range(reps):
+        fn()
+    e1.record(None)
+    abi.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def _row(rows, kernel, config, ms, nbytes=None, flops=None, bound="hbm", peak=None, **extra):
+    r = {"kernel": kernel, "config": config, "ms_per_rep": round(ms, 5)}
+    if nbytes is not None:
+        gbs = nbytes / (ms / 1e3) / 1e9
+        r.update({"GB/s": round(gbs, 1)})
+        if bound == "hbm":
+            r.update({"bound": "hbm", "frac": round(gbs / HBM_GBS, 4)})
+    if flops is not None:
+        tf = flops / (ms / 1e3) / 1e12
+        r.update({"TFLOP/s": round(tf, 2)})
+        if bound != "hbm":
+            r.update({"bound": bound, "frac": round(tf / peak, 4)})
+    r.update(extra)
+    rows.append(r)
+    print("  ".join(f"{k}={v}" for k, v in r.items()), flush=True)
+
+
+def config2(rows, reps):
+    L = abi.lib()
+    n = 1 << 28
+    B = abi.DeviceBuffer
+    a, b, c = B(8 * n), B(8 * n), B(8 * n)
+    L.rpx_fill_uniform_f64(b.ptr, n, SEED, 1, 0.0, 1.0, None)
+    L.rpx_fill_uniform_f64(c.ptr, n, SEED, 2, 0.0, 1.0, None)
+    _row(rows, "Stream_COPY", 2, _time(lambda: L.rpx_copy_f64(a.ptr, b.ptr, n, None, None), reps), 16 * n)
+    _row(rows, "Stream_MUL", 2, _time(lambda: L.rpx_mul_f64(a.ptr, b.ptr, 0.75, n, None, None), reps), 16 * n)
+    _row(rows, "Stream_ADD", 2, _time(lambda: L.rpx_add_f64(a.ptr, b.ptr, c.ptr, n, None, None), reps), 24 * n)
+    _row(rows, "Stream_TRIAD", 2, _time(lambda: L.rpx_triad_f64(a.ptr, b.ptr, c.ptr, 0.75, n, None, None), reps),
+         24 * n)
+    sb = L.rpx_reduce_scratch_bytes(n, None)
+    scr, res = B(sb), B(16)
+    L.rpx_reduce_scratch_init(scr.ptr, sb, None)
+    _row(rows, "Stream_DOT", 2,
+         _time(lambda: L.rpx_dot_f64(b.ptr, c.ptr, n, res.ptr, 0, scr.ptr, sb, None, None), reps), 16 * n + 16)
+    _row(rows, "Algorithm_REDUCE_SUM", 2,
+         _time(lambda: L.rpx_reduce_sum_f64(b.ptr, n, res.ptr, scr.ptr, sb, None, None), reps), 8 * n + 16)
+    ssb = L.rpx_scan_scratch_bytes(n, None)
+    sscr = B(ssb)
+    L.rpx_scan_scratch_init(sscr.ptr, ssb, None)
+    _row(rows, "Algorithm_SCAN", 2,
+         _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, None, None), reps), 16 * n)
+    del a, b, c
+    N = 8192
+    gA, gB = B(8 * N * N), B(8 * N * N)
+    L.rpx_fill_grid_f64(gA.ptr, N, SEED, 0, 0.0, 1.0, 0.0, None)
+    L.rpx_fill_grid_f64(gB.ptr, N, SEED, 1, 0.0, 1.0, 0.0, None)
+    sweep = 8 * (N * N - 4) + 8 * (N - 2) * (N - 2)
+    ms = _time(lambda: L.rpx_jacobi2d_f64(gA.ptr, gB.ptr, N, 50, None, None), max(1, reps // 2))
+    _row(rows, "Polybench_JACOBI_2D(sweep)", 2, ms / 100, sweep)
+
+
+def config3(rows, reps):
+    L = abi.lib()
+    NE, D, Q = 262144, 4, 5
+    B = abi.DeviceBuffer
+    Bm, Bt = B(8 * Q * D), B(8 * Q * D)
+    L.rpx_fill_uniform_f64(Bm.ptr, Q * D, SEED, 0, -1.0, 1.0, None)
+    L.rpx_fill_uniform_f64(Bt.ptr, Q * D, SEED, 1, -1.0, 1.0, None)
+    Dm = B(8 * NE * Q ** 3)
+    X, Y = B(8 * NE * D ** 3), B(8 * NE * D ** 3)
+    L.rpx_fill_uniform_f64(Dm.ptr, NE * Q ** 3, SEED, 2, 0.5, 2.0, None)
+    L.rpx_fill_uniform_f64(X.ptr, NE * D ** 3, SEED, 3, -1.0, 1.0, None)
+    L.rpx_fill_const_f64(Y.ptr, NE * D ** 3, 0.0, None)
+    nbytes = 2 * 8 * Q * D + 2 * 8 * D ** 3 * NE + 8 * Q ** 3 * NE + 8 * D ** 3 * NE
+    flops = NE * (2 * D ** 3 * Q + 2 * D * D * Q * Q + 2 * D * Q ** 3 + Q ** 3 + 2 * Q ** 3 * D + 2 * Q * Q * D * D
+                  + 2 * Q * D ** 3 + D ** 3)
+    ms = _time(lambda: L.rpx_mass3dpa_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Y.ptr, NE, D, Q, None, None), reps)
+    _row(rows, "Apps_MASS3DPA", 3, ms, nbytes, flops)
+    del Dm
+    Dd = B(8 * NE * 6 * Q ** 3)
+    L.rpx_fill_sym_tensor_f64(Dd.ptr, NE, Q ** 3, 6, SEED, 2, None)
+    nbytes = 2 * 8 * Q * D + 2 * 8 * D ** 3 * NE + 6 * 8 * Q ** 3 * NE + 8 * D ** 3 * NE
+    flops = NE * (Q * D + 5 * D * D * Q * D + 7 * D * D * Q * Q + 7 * Q * D * Q * Q + 15 * Q ** 3 + Q * D
+                  + 7 * Q * Q * D * Q + 7 * Q * Q * D * D + 7 * D * Q * D * D + 3 * D ** 3)
+    ms = _time(lambda: L.rpx_diffusion3dpa_f64(Bm.ptr, Bt.ptr, Dd.ptr, X.ptr, Y.ptr, NE, D, Q, 1, None, None), reps)
+    _row(rows, "Apps_DIFFUSION3DPA", 3, ms, nbytes, flops)
+
+
+def config4(rows, reps, sample_rows=8):
+    L = abi.lib()
+    n = 4096
+    B = abi.DeviceBuffer
+    dA, dB = B(8 * n * n), B(8 * n * n)
+    L.rpx_fill_uniform_f64(dA.ptr, n * n, SEED, 0, -1.0, 1.0, None)
+    L.rpx_fill_uniform_f64(dB.ptr, n * n, SEED, 1, -1.0, 1.0, None)
+    Cf, Cm = B(8 * n * n), B(8 * n * n)
+    alpha = 1.5
+    flops = 2 * n ** 3
+    ms_f64 = _time(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None), reps)
+    naive = abi.tuning(flags=abi.RPX_TUNE_NAIVE)
+    ms_naive = _time(lambda: L.rpx_gemm_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, naive, None), 1, warmup=1)
+    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 7)
+    scr = B(sb)
+    ms_mix = _time(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 7, scr.ptr, sb, None),
+                   reps)
+    # error on sampled rows against 80-bit host arithmetic
+    A = dA.to_array().reshape(n, n)
+    Bh = dB.to_array().reshape(n, n)
+    rng = np.random.default_rng(0)
+    rows_idx = np.sort(rng.choice(n, sample_rows, replace=False))
+    Cmix = Cm.to_array().reshape(n, n)[rows_idx]
+    C64 = Cf.to_array().reshape(n, n)[rows_idx]
+    Al = A[rows_idx].astype(np.longdouble)
+    ref = (Al @ Bh.astype(np.longdouble)) * np.longdouble(alpha)
+    scale = alpha * (np.abs(A[rows_idx]) @ np.abs(Bh))
+    err_mix = float(np.max(np.abs(Cmix - ref) / scale))
+    err_f64 = float(np.max(np.abs(C64 - ref) / scale))
+    _row(rows, "Polybench_GEMM(fp64-mfma)", 4, ms_f64, None, flops, "mfma", FP64_TFLOPS, err_vs_exact=err_f64)
+    _row(rows, "Polybench_GEMM(fp64-naive)", 4, ms_naive, None, flops, "mfma", FP64_TFLOPS)
+    _row(rows, "Polybench_GEMM(mixed-int8x7)", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
+         err_vs_exact=err_mix, int8_tops=round(28 * flops / (ms_mix / 1e3) / 1e12, 1),
+         speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
+
+
+def config5(rows, reps):
+    L = abi.lib()
+    rng = np.random.default_rng(SEED)
+    lens = np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 20), 256)).astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(lens)])
+    n = int(offs[-1])
+    B = abi.DeviceBuffer
+    src, x, y = B(8 * n), B(8 * n), B(8 * n)
+    L.rpx_fill_uniform_f64(src.ptr, n, SEED, 0, -1.0, 1.0, None)
+    o = abi.offsets_array(offs.tolist())
+    sb = L.rpx_sort_segmented_scratch_bytes(o, 256)
+    scr = B(sb)
+
+    def sort_once():
+        L.rpx_memcpy_d2d(x.ptr, src.ptr, 8 * n, None)
+        L.rpx_sort_segmented_f64(x.ptr, o, 256, scr.ptr, sb, None)
+    ms_copy = _time(lambda: L.rpx_memcpy_d2d(x.ptr, src.ptr, 8 * n, None), reps)
+    ms = _time(sort_once, reps) - ms_copy
+    _row(rows, "Algorithm_SORT(256 ragged)", 5, ms, 8 * 16 * n, None, keys=n,
+         Gkeys_per_s=round(n / (ms / 1e3) / 1e9, 3))
+    ssb = L.rpx_scan_segmented_scratch_bytes(o, 256)
+    sscr = B(ssb)
+    L.rpx_scan_scratch_init(sscr.ptr, ssb, None)
+    ms = _time(lambda: L.rpx_scan_exclusive_segmented_f64(src.ptr, y.ptr, o, 256, sscr.ptr, ssb, None), reps)
+    _row(rows, "Algorithm_SCAN(256 ragged)", 5, ms, 16 * n)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--configs", default="2,3,4,5")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--csv", default=None)
+    args = ap.parse_args(argv)
+    arch = abi.init(0)
+    print(f"device {arch}", flush=True)
+    rows = []
+    for c in args.configs.split(","):
+        {"2": config2, "3": config3, "4": config4, "5": config5}[c.strip()](rows, args.reps)
+    if args.csv:
+        keys = sorted({k for r in rows for k in r})
+        with open(args.csv, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=keys)
+            w.writeheader()
+            w.writerows(rows)
+    return rows
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
