@@ -196,6 +196,16 @@ int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N,
  * (POLYBENCH_JACOBI_2D-Seq.cpp:30-54, -Hip.cpp:27-114). */
 int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
                      const rpx_tuning* tune, void* stream);
+/* Same result, with each time step (both sweeps) fused into one launch: A is
+ * read once and B and the new A written once per step, the new A going to
+ * `scratch` (an N x N grid) and back, so no launch overwrites values a
+ * neighbouring tile still reads.  Ends with the result in A and B, bit-equal
+ * to rpx_jacobi2d_f64.  Falls back to the two-sweep path for odd N or
+ * pointers that are not 16-byte aligned. */
+size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
+int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps,
+                           void* scratch, size_t scratch_bytes,
+                           const rpx_tuning* tune, void* stream);
 
 /* GEMM with the suite's semantics (src/polybench/POLYBENCH_GEMM.hpp:12-50):
  *   C[i][j] = sum_k (alpha*A[i][k])*B[k][j]

@@ -95,6 +95,8 @@ _SIGS = {
     "rpx_sort_segmented_f64": (_i32, [_vp, _i64p, _i64, _vp, _sz, _vp]),
     "rpx_jacobi2d_sweep_f64": (_i32, [_vp, _vp, _i64, _tp, _vp]),
     "rpx_jacobi2d_f64": (_i32, [_vp, _vp, _i64, _i64, _tp, _vp]),
+    "rpx_jacobi2d_fused_scratch_bytes": (_sz, [_i64]),
+    "rpx_jacobi2d_fused_f64": (_i32, [_vp, _vp, _i64, _i64, _vp, _sz, _tp, _vp]),
     "rpx_gemm_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _tp, _vp]),
     "rpx_gemm_mixed_scratch_bytes": (_sz, [_i64, _i64, _i64, _i32]),
     "rpx_gemm_mixed_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _i32, _vp, _sz, _vp]),

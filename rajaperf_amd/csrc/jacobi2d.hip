@@ -113,7 +113,140 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
   }
 }
 
+// ---- fused time step: A -> B -> A' in one launch ------------------------------
+//
+// One wavefront owns a strip of 128 columns starting at j0 = 124*s - 2 (its
+// lanes hold column pairs) and a chunk of R output rows.  It loads R+4 rows
+// of A (the level-0 input, two halo rows above and below), computes the
+// level-1 rows (the B of the time step) one at a time into a 3-row rolling
+// window, and from that window the level-2 rows (the new A).  West/east
+// neighbours at both levels come from the adjacent lanes; the two columns at
+// each strip edge are halo, so a strip writes the 124 columns
+// [j0+2, j0+126).  Level-1 values on the grid boundary are B's boundary
+// values, as in the two-sweep update.  B's interior and A' are written by
+// exactly one wavefront each; A' goes to a different grid than A, so no
+// wavefront reads a value another one has already replaced.
+constexpr int kStripValid = 124;  // 128 columns per wavefront strip, 2 halo columns per side
+
+template <int R>
+__global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ Aout, double* __restrict__ B,
+                                                           const double* __restrict__ Ain, int64_t N, int strips,
+                                                           int chunks) {
+  const int lane = threadIdx.x & (kWave - 1);
+  // XCD-banded workgroup order (see logical_tile): each XCD walks a band of
+  // row chunks, so the halo rows shared by vertically adjacent chunks are
+  // re-read from the same L2.
+  const unsigned wave_global =
+      logical_tile(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave);
+  const unsigned total = unsigned(strips) * unsigned(chunks);
+  if (wave_global >= total) return;
+  const unsigned tile = wave_global;  // strip fastest: a workgroup spans 4 adjacent strips
+  const int strip = int(tile % unsigned(strips));
+  const int chunk = int(tile / unsigned(strips));
+  const int64_t j0 = int64_t(strip) * kStripValid - 2;
+  const int64_t jl = j0 + 2 * lane;  // first column of this lane's pair
+  const bool col_in = jl >= 0 && jl < N;  // N even and jl even: jl+1 < N too
+  const int64_t r0 = 1 + int64_t(chunk) * R;
+  int64_t rend = r0 + R;
+  if (rend > N - 1) rend = N - 1;
+  const int rows = int(rend - r0);
+
+  // Level 0: A rows r0-2 .. r0+R+1.
+  v2d a[R + 4];
+#pragma unroll
+  for (int k = 0; k < R + 4; ++k) {
+    const int64_t row = r0 - 2 + k;
+    a[k] = v2d{0.0, 0.0};
+    if (col_in && row >= 0 && row < N && k <= rows + 3) a[k] = *reinterpret_cast<const v2d*>(Ain + row * N + jl);
+  }
+  // B boundary values for level-1 points on the grid boundary.
+  auto level1 = [&](int k) -> v2d {  // level-1 row r0-1+k from level-0 rows k..k+2
+    const int64_t row = r0 - 1 + k;
+    const v2d c = a[k + 1];
+    double w = __shfl_up(c.y, 1, kWave);
+    double e = __shfl_down(c.x, 1, kWave);
+    v2d r;
+    r.x = point(c.x, w, c.y, a[k + 2].x, a[k].x);
+    r.y = point(c.y, c.x, e, a[k + 2].y, a[k].y);
+    if (col_in && row >= 0 && row < N) {
+      if (row == 0 || row == N - 1) {
+        r = *reinterpret_cast<const v2d*>(B + row * N + jl);
+      } else {
+        if (jl == 0) r.x = B[row * N];
+        if (jl + 1 == N - 1) r.y = B[row * N + N - 1];
+      }
+    }
+    return r;
+  };
+
+  const bool x_out = col_in && jl >= j0 + 2 && jl < j0 + 126 && jl >= 1 && jl <= N - 2;
+  const bool y_out = col_in && jl + 1 >= j0 + 2 && jl + 1 < j0 + 126 && jl + 1 >= 1 && jl + 1 <= N - 2;
+  v2d l_prev = level1(0), l_cur = level1(1);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const v2d l_next = level1(k + 2);
+    // level 2 at row r0+k from level-1 rows r0+k-1 .. r0+k+1
+    const double w = __shfl_up(l_cur.y, 1, kWave);
+    const double e = __shfl_down(l_cur.x, 1, kWave);
+    if (k < rows) {
+      const int64_t row = r0 + k;
+      const double o0 = point(l_cur.x, w, l_cur.y, l_next.x, l_prev.x);
+      const double o1 = point(l_cur.y, l_cur.x, e, l_next.y, l_prev.y);
+      double* bo = B + row * N + jl;
+      double* ao = Aout + row * N + jl;
+      if (x_out && y_out) {
+        *reinterpret_cast<v2d*>(bo) = l_cur;
+        *reinterpret_cast<v2d*>(ao) = v2d{o0, o1};
+      } else {
+        if (x_out) {
+          bo[0] = l_cur.x;
+          ao[0] = o0;
+        }
+        if (y_out) {
+          bo[1] = l_cur.y;
+          ao[1] = o1;
+        }
+      }
+    }
+    l_prev = l_cur;
+    l_cur = l_next;
+  }
+}
+
+// Copies the boundary rows and columns of src into dst.
+__global__ __launch_bounds__(256) void boundary_copy_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                                                            int64_t N) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < N) {
+    dst[t] = src[t];                                  // row 0
+    dst[(N - 1) * N + t] = src[(N - 1) * N + t];      // row N-1
+    dst[t * N] = src[t * N];                          // column 0
+    dst[t * N + N - 1] = src[t * N + N - 1];          // column N-1
+  }
+}
+
+template <int R>
+int launch_fused(double* Aout, double* B, const double* Ain, int64_t N, hipStream_t s) {
+  const int64_t strips = ceil_div(N + 2, kStripValid);  // covers columns [0, N)
+  const int64_t chunks = ceil_div(N - 2, R);
+  const int64_t waves = strips * chunks;
+  const int64_t grid = ceil_div(waves, 4);
+  if (grid > INT32_MAX || strips > INT32_MAX || chunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  jacobi_fused_kernel<R><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, int(strips), int(chunks));
+  return launch_status();
+}
+
+int fused_step(double* Aout, double* B, const double* Ain, int64_t N, int rows, hipStream_t s) {
+  switch (rows) {
+    case 8: return launch_fused<8>(Aout, B, Ain, N, s);
+    case 16: return launch_fused<16>(Aout, B, Ain, N, s);
+    case 32: return launch_fused<32>(Aout, B, Ain, N, s);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
 constexpr int kDefaultRows = 16;
+constexpr int kDefaultFusedRows = 16;
 
 template <int R>
 int launch_vec(double* dst, const double* src, int64_t N, hipStream_t s) {
@@ -150,6 +283,40 @@ extern "C" {
 int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N, const rpx_tuning* tune, void* stream) {
   if (N < 0 || (N > 0 && (!dst || !src)) || dst == src) return RPX_ERR_INVALID_ARGUMENT;
   return sweep(dst, src, N, tune, as_stream(stream));
+}
+
+size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N) { return N > 0 ? size_t(N) * size_t(N) * sizeof(double) : 0; }
+
+int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void* scratch, size_t scratch_bytes,
+                           const rpx_tuning* tune, void* stream) {
+  if (N < 0 || tsteps < 0 || (N > 0 && (!A || !B)) || A == B) return RPX_ERR_INVALID_ARGUMENT;
+  if (N < 3 || tsteps == 0) return RPX_SUCCESS;
+  hipStream_t s = as_stream(stream);
+  const bool vec = (N % 2 == 0) && aligned16(A) && aligned16(B) && aligned16(scratch);
+  if (!vec) return rpx_jacobi2d_f64(A, B, N, tsteps, tune, stream);
+  if (!scratch || scratch == A || scratch == B) return RPX_ERR_INVALID_ARGUMENT;
+  if (scratch_bytes < rpx_jacobi2d_fused_scratch_bytes(N)) return RPX_ERR_SCRATCH_TOO_SMALL;
+  const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
+  if (rows != 8 && rows != 16 && rows != 32) return RPX_ERR_UNSUPPORTED;
+  double* Q = reinterpret_cast<double*>(scratch);
+  int64_t t = 0;
+  if (tsteps % 2 == 1) {  // one two-sweep step first, then pairs of fused steps end in A
+    int rc = sweep(B, A, N, nullptr, s);
+    if (rc == RPX_SUCCESS) rc = sweep(A, B, N, nullptr, s);
+    if (rc != RPX_SUCCESS) return rc;
+    t = 1;
+  }
+  if (t < tsteps) {
+    boundary_copy_kernel<<<dim3(unsigned(ceil_div(N, 256))), dim3(256), 0, s>>>(Q, A, N);
+    const int rc = launch_status();
+    if (rc != RPX_SUCCESS) return rc;
+  }
+  for (; t < tsteps; t += 2) {
+    int rc = fused_step(Q, B, A, N, rows, s);
+    if (rc == RPX_SUCCESS) rc = fused_step(A, B, Q, N, rows, s);
+    if (rc != RPX_SUCCESS) return rc;
+  }
+  return RPX_SUCCESS;
 }
 
 int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps, const rpx_tuning* tune, void* stream) {

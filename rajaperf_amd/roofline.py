@@ -95,7 +95,11 @@ def config2(rows, reps):
     L.rpx_fill_grid_f64(gB.ptr, N, SEED, 1, 0.0, 1.0, 0.0, None)
     sweep = 8 * (N * N - 4) + 8 * (N - 2) * (N - 2)
     ms = _time(lambda: L.rpx_jacobi2d_f64(gA.ptr, gB.ptr, N, 50, None, None), max(1, reps // 2))
-    _row(rows, "Polybench_JACOBI_2D(sweep)", 2, ms / 100, sweep)
+    _row(rows, "Polybench_JACOBI_2D(two-sweep, per sweep)", 2, ms / 100, sweep)
+    jsb = L.rpx_jacobi2d_fused_scratch_bytes(N)
+    gQ = B(jsb)
+    ms = _time(lambda: L.rpx_jacobi2d_fused_f64(gA.ptr, gB.ptr, N, 50, gQ.ptr, jsb, None, None), max(1, reps // 2))
+    _row(rows, "Polybench_JACOBI_2D(fused, per time step)", 2, ms / 50, 2 * sweep)
 
 
 def config3(rows, reps):

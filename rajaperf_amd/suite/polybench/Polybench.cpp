@@ -79,9 +79,10 @@ POLYBENCH_JACOBI_2D::POLYBENCH_JACOBI_2D(const RunParams& params) : KernelBase(P
   setVariantDefined(MI355X_Native);
 }
 void POLYBENCH_JACOBI_2D::setMI355XTuningDefinitions(VariantID vid) {
-  addVariantTuningName(vid, "default");
-  addVariantTuningName(vid, "rows_8");
-  addVariantTuningName(vid, "rows_32");
+  addVariantTuningName(vid, "default");   // one fused launch per time step, 16-row chunks
+  addVariantTuningName(vid, "fused_rows_32");
+  addVariantTuningName(vid, "two_sweep");  // one launch per sweep, as the suite's kernels
+  addVariantTuningName(vid, "two_sweep_rows_32");
 }
 void POLYBENCH_JACOBI_2D::setUp(VariantID, size_t) {
   const Size_type nn = Size_type(m_N * m_N);
@@ -110,16 +111,21 @@ void POLYBENCH_JACOBI_2D::tearDown(VariantID, size_t) {
 void POLYBENCH_JACOBI_2D::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
   rpx_tuning tune = defaultTuning();
-  if (tune_idx == 1) tune.items_per_thread = 8;
-  if (tune_idx == 2) tune.items_per_thread = 32;
+  const bool fused = tune_idx <= 1;
+  if (tune_idx == 1 || tune_idx == 3) tune.items_per_thread = 32;
   // POLYBENCH_JACOBI_2D_DATA_SETUP copies the initial grids before the timer.
   const Size_type bytes = Size_type(m_N * m_N) * sizeof(Real_type);
   rpxCheck(rpx_memcpy_d2d(m_A, m_Ainit, bytes, nullptr), "JACOBI_2D copy");
   rpxCheck(rpx_memcpy_d2d(m_B, m_Binit, bytes, nullptr), "JACOBI_2D copy");
+  const size_t sb = fused ? rpx_jacobi2d_fused_scratch_bytes(m_N) : 0;
+  void* scratch = sb ? detail::deviceAlloc(sb) : nullptr;
   startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_jacobi2d_f64(m_A, m_B, m_N, m_tsteps, &tune, nullptr), "JACOBI_2D");
+  for (RepIndex_type irep = 0; irep < reps; ++irep) {
+    if (fused) rpxCheck(rpx_jacobi2d_fused_f64(m_A, m_B, m_N, m_tsteps, scratch, sb, &tune, nullptr), "JACOBI_2D");
+    else rpxCheck(rpx_jacobi2d_f64(m_A, m_B, m_N, m_tsteps, &tune, nullptr), "JACOBI_2D");
+  }
   stopTimer();
+  if (scratch) detail::deviceFree(scratch);
 }
 
 }  // namespace polybench
