@@ -55,6 +55,8 @@ typedef struct rpx_tuning {
   int32_t items_per_thread;  /* elements per work-item per grid step        */
   int32_t grid_size;         /* workgroups; 0 = derived from the size       */
   int32_t flags;             /* RPX_TUNE_* bits                             */
+  int32_t depth;             /* temporal blocking: JACOBI_2D time steps per
+                                fused launch (1..3)                          */
 } rpx_tuning;
 
 #define RPX_TUNE_NONTEMPORAL_STORE 0x1 /* streaming stores bypass the cache */
@@ -196,13 +198,25 @@ int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N,
  * (POLYBENCH_JACOBI_2D-Seq.cpp:30-54, -Hip.cpp:27-114). */
 int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
                      const rpx_tuning* tune, void* stream);
-/* Same result, with each time step (both sweeps) fused into one launch: A is
- * read once and B and the new A written once per step, the new A going to
- * `scratch` (an N x N grid) and back, so no launch overwrites values a
+/* Same result with temporal blocking: each launch runs `depth` whole time
+ * steps (2*depth sweeps, tune->depth, default 2) on register-resident tiles,
+ * reading A once and writing only the final B and the new A, the new A going
+ * to `scratch` (an N x N grid) and back, so no launch overwrites values a
  * neighbouring tile still reads.  Ends with the result in A and B, bit-equal
  * to rpx_jacobi2d_f64.  Falls back to the two-sweep path for odd N or
  * pointers that are not 16-byte aligned. */
 size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
+/* The building block of rpx_jacobi2d_fused_f64: one launch advancing `depth`
+ * time steps (1..3) from Ain into Aout, writing the final B.  Requires even
+ * N, 16-byte aligned grids, Aout != Ain, and Aout's boundary equal to Ain's
+ * (copy it once with rpx_jacobi2d_copy_boundary_f64); callers ping-pong Ain
+ * and Aout between launches. */
+int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain,
+                                  int64_t N, int depth,
+                                  const rpx_tuning* tune, void* stream);
+/* dst's boundary rows and columns := src's. */
+int rpx_jacobi2d_copy_boundary_f64(double* dst, const double* src,
+                                   int64_t N, void* stream);
 int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps,
                            void* scratch, size_t scratch_bytes,
                            const rpx_tuning* tune, void* stream);

@@ -44,7 +44,7 @@ class RpxError(RuntimeError):
 
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("items_per_thread", C.c_int32),
-                ("grid_size", C.c_int32), ("flags", C.c_int32)]
+                ("grid_size", C.c_int32), ("flags", C.c_int32), ("depth", C.c_int32)]
 
 
 _vp, _i64, _u64, _d, _i32, _sz = C.c_void_p, C.c_int64, C.c_uint64, C.c_double, C.c_int, C.c_size_t
@@ -97,6 +97,8 @@ _SIGS = {
     "rpx_jacobi2d_f64": (_i32, [_vp, _vp, _i64, _i64, _tp, _vp]),
     "rpx_jacobi2d_fused_scratch_bytes": (_sz, [_i64]),
     "rpx_jacobi2d_fused_f64": (_i32, [_vp, _vp, _i64, _i64, _vp, _sz, _tp, _vp]),
+    "rpx_jacobi2d_fused_launch_f64": (_i32, [_vp, _vp, _vp, _i64, _i32, _tp, _vp]),
+    "rpx_jacobi2d_copy_boundary_f64": (_i32, [_vp, _vp, _i64, _vp]),
     "rpx_gemm_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _tp, _vp]),
     "rpx_gemm_mixed_scratch_bytes": (_sz, [_i64, _i64, _i64, _i32]),
     "rpx_gemm_mixed_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _i32, _vp, _sz, _vp]),
@@ -164,8 +166,8 @@ def lib() -> _Lib:
     return _LIB
 
 
-def tuning(block_size=0, items_per_thread=0, grid_size=0, flags=0):
-    return C.pointer(Tuning(block_size, items_per_thread, grid_size, flags))
+def tuning(block_size=0, items_per_thread=0, grid_size=0, flags=0, depth=0):
+    return C.pointer(Tuning(block_size, items_per_thread, grid_size, flags, depth))
 
 
 # ---- device helpers ---------------------------------------------------------

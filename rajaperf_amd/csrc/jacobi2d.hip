@@ -113,25 +113,30 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
   }
 }
 
-// ---- fused time step: A -> B -> A' in one launch ------------------------------
+// ---- fused time steps: T steps (2T sweeps) per launch -----------------------
 //
-// One wavefront owns a strip of 128 columns starting at j0 = 124*s - 2 (its
-// lanes hold column pairs) and a chunk of R output rows.  It loads R+4 rows
-// of A (the level-0 input, two halo rows above and below), computes the
-// level-1 rows (the B of the time step) one at a time into a 3-row rolling
-// window, and from that window the level-2 rows (the new A).  West/east
-// neighbours at both levels come from the adjacent lanes; the two columns at
-// each strip edge are halo, so a strip writes the 124 columns
-// [j0+2, j0+126).  Level-1 values on the grid boundary are B's boundary
-// values, as in the two-sweep update.  B's interior and A' are written by
-// exactly one wavefront each; A' goes to a different grid than A, so no
-// wavefront reads a value another one has already replaced.
-constexpr int kStripValid = 124;  // 128 columns per wavefront strip, 2 halo columns per side
+// One wavefront owns a strip of 128 columns (its lanes hold column pairs) and
+// a chunk of R output rows.  With H = 2T sweeps per launch the strip starts at
+// j0 = (128-2H)*s - H and writes the 128-2H columns [j0+H, j0+128-H); the H
+// columns at each edge are halo.  The wavefront loads R+2H rows of A (level 0)
+// and runs a cascade of levels 1..2T: at step q, level L produces its row
+// rbase+q-L from the last three rows of level L-1, held in a 3-row rolling
+// register window.  West/east neighbours come from the adjacent lanes.  Odd
+// levels are B's of the time steps, even levels the new A's; on the grid
+// boundary a level takes B's boundary values (odd levels) or A's (even
+// levels, which never change), exactly as the two-sweep update reads them.
+// Level 2T-1 (the final B) and level 2T (the final A) are written for the
+// owned rows and columns only, each by exactly one wavefront; the new A goes
+// to a different grid than the one read, so no wavefront reads a value
+// another one has already replaced.
+constexpr int kStripCols = 128;
 
-template <int R>
+template <int R, int T>
 __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ Aout, double* __restrict__ B,
                                                            const double* __restrict__ Ain, int64_t N, int strips,
                                                            int chunks) {
+  constexpr int H = 2 * T;       // sweeps per launch = halo width
+  constexpr int NR = R + 2 * H;  // level-0 rows held
   const int lane = threadIdx.x & (kWave - 1);
   // XCD-banded workgroup order (see logical_tile): each XCD walks a band of
   // row chunks, so the halo rows shared by vertically adjacent chunks are
@@ -140,76 +145,74 @@ __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ 
       logical_tile(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave);
   const unsigned total = unsigned(strips) * unsigned(chunks);
   if (wave_global >= total) return;
-  const unsigned tile = wave_global;  // strip fastest: a workgroup spans 4 adjacent strips
-  const int strip = int(tile % unsigned(strips));
-  const int chunk = int(tile / unsigned(strips));
-  const int64_t j0 = int64_t(strip) * kStripValid - 2;
-  const int64_t jl = j0 + 2 * lane;  // first column of this lane's pair
+  const int strip = int(wave_global % unsigned(strips));  // a workgroup spans 4 adjacent strips
+  const int chunk = int(wave_global / unsigned(strips));
+  const int64_t j0 = int64_t(strip) * (kStripCols - 2 * H) - H;
+  const int64_t jl = j0 + 2 * lane;       // first column of this lane's pair
   const bool col_in = jl >= 0 && jl < N;  // N even and jl even: jl+1 < N too
   const int64_t r0 = 1 + int64_t(chunk) * R;
   int64_t rend = r0 + R;
   if (rend > N - 1) rend = N - 1;
   const int rows = int(rend - r0);
+  const int64_t rbase = r0 - H;  // global row of level-0 index 0
 
-  // Level 0: A rows r0-2 .. r0+R+1.
-  v2d a[R + 4];
+  v2d a[NR];
 #pragma unroll
-  for (int k = 0; k < R + 4; ++k) {
-    const int64_t row = r0 - 2 + k;
+  for (int k = 0; k < NR; ++k) {
+    const int64_t row = rbase + k;
     a[k] = v2d{0.0, 0.0};
-    if (col_in && row >= 0 && row < N && k <= rows + 3) a[k] = *reinterpret_cast<const v2d*>(Ain + row * N + jl);
+    if (col_in && row >= 0 && row < N && k < rows + 2 * H) a[k] = *reinterpret_cast<const v2d*>(Ain + row * N + jl);
   }
-  // B boundary values for level-1 points on the grid boundary.
-  auto level1 = [&](int k) -> v2d {  // level-1 row r0-1+k from level-0 rows k..k+2
-    const int64_t row = r0 - 1 + k;
-    const v2d c = a[k + 1];
-    double w = __shfl_up(c.y, 1, kWave);
-    double e = __shfl_down(c.x, 1, kWave);
-    v2d r;
-    r.x = point(c.x, w, c.y, a[k + 2].x, a[k].x);
-    r.y = point(c.y, c.x, e, a[k + 2].y, a[k].y);
-    if (col_in && row >= 0 && row < N) {
-      if (row == 0 || row == N - 1) {
-        r = *reinterpret_cast<const v2d*>(B + row * N + jl);
-      } else {
-        if (jl == 0) r.x = B[row * N];
-        if (jl + 1 == N - 1) r.y = B[row * N + N - 1];
-      }
-    }
-    return r;
-  };
 
-  const bool x_out = col_in && jl >= j0 + 2 && jl < j0 + 126 && jl >= 1 && jl <= N - 2;
-  const bool y_out = col_in && jl + 1 >= j0 + 2 && jl + 1 < j0 + 126 && jl + 1 >= 1 && jl + 1 <= N - 2;
-  v2d l_prev = level1(0), l_cur = level1(1);
+  const bool x_out = col_in && jl >= j0 + H && jl < j0 + kStripCols - H && jl >= 1 && jl <= N - 2;
+  const bool y_out = col_in && jl + 1 >= j0 + H && jl + 1 < j0 + kStripCols - H && jl + 1 >= 1 && jl + 1 <= N - 2;
+
+  v2d win[H][3];  // win[L-1]: the last three rows of level L
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const v2d l_next = level1(k + 2);
-    // level 2 at row r0+k from level-1 rows r0+k-1 .. r0+k+1
-    const double w = __shfl_up(l_cur.y, 1, kWave);
-    const double e = __shfl_down(l_cur.x, 1, kWave);
-    if (k < rows) {
-      const int64_t row = r0 + k;
-      const double o0 = point(l_cur.x, w, l_cur.y, l_next.x, l_prev.x);
-      const double o1 = point(l_cur.y, l_cur.x, e, l_next.y, l_prev.y);
-      double* bo = B + row * N + jl;
-      double* ao = Aout + row * N + jl;
-      if (x_out && y_out) {
-        *reinterpret_cast<v2d*>(bo) = l_cur;
-        *reinterpret_cast<v2d*>(ao) = v2d{o0, o1};
-      } else {
-        if (x_out) {
-          bo[0] = l_cur.x;
-          ao[0] = o0;
+  for (int q = 2; q < NR; ++q) {
+#pragma unroll
+    for (int L = 1; L <= H; ++L) {
+      if (q < 2 * L) continue;  // level L starts once level L-1 has three rows
+      const v2d up = (L == 1) ? a[q - 2] : win[L - 2][0];
+      const v2d c = (L == 1) ? a[q - 1] : win[L - 2][1];
+      const v2d dn = (L == 1) ? a[q] : win[L - 2][2];
+      const double w = __shfl_up(c.y, 1, kWave);
+      const double e = __shfl_down(c.x, 1, kWave);
+      v2d r;
+      r.x = point(c.x, w, c.y, dn.x, up.x);
+      r.y = point(c.y, c.x, e, dn.y, up.y);
+      const int64_t g = rbase + q - L;  // global row of this level row
+      if (col_in && g >= 0 && g < N) {
+        if (L % 2 == 1) {  // a B level: B's boundary values
+          if (g == 0 || g == N - 1) {
+            r = *reinterpret_cast<const v2d*>(B + g * N + jl);
+          } else {
+            if (jl == 0) r.x = B[g * N];
+            if (jl + 1 == N - 1) r.y = B[g * N + N - 1];
+          }
+        } else {  // an A level: A's boundary values, held at level 0
+          const v2d a0 = a[q - L];
+          if (g == 0 || g == N - 1) {
+            r = a0;
+          } else {
+            if (jl == 0) r.x = a0.x;
+            if (jl + 1 == N - 1) r.y = a0.y;
+          }
         }
-        if (y_out) {
-          bo[1] = l_cur.y;
-          ao[1] = o1;
+      }
+      win[L - 1][0] = win[L - 1][1];
+      win[L - 1][1] = win[L - 1][2];
+      win[L - 1][2] = r;
+      if ((L == H - 1 || L == H) && g >= r0 && g < rend) {
+        double* out = (L == H ? Aout : B) + g * N + jl;
+        if (x_out && y_out) {
+          *reinterpret_cast<v2d*>(out) = r;
+        } else {
+          if (x_out) out[0] = r.x;
+          if (y_out) out[1] = r.y;
         }
       }
     }
-    l_prev = l_cur;
-    l_cur = l_next;
   }
 }
 
@@ -218,35 +221,48 @@ __global__ __launch_bounds__(256) void boundary_copy_kernel(double* __restrict__
                                                             int64_t N) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t < N) {
-    dst[t] = src[t];                                  // row 0
-    dst[(N - 1) * N + t] = src[(N - 1) * N + t];      // row N-1
-    dst[t * N] = src[t * N];                          // column 0
-    dst[t * N + N - 1] = src[t * N + N - 1];          // column N-1
+    dst[t] = src[t];                              // row 0
+    dst[(N - 1) * N + t] = src[(N - 1) * N + t];  // row N-1
+    dst[t * N] = src[t * N];                      // column 0
+    dst[t * N + N - 1] = src[t * N + N - 1];      // column N-1
   }
 }
 
-template <int R>
+template <int R, int T>
 int launch_fused(double* Aout, double* B, const double* Ain, int64_t N, hipStream_t s) {
-  const int64_t strips = ceil_div(N + 2, kStripValid);  // covers columns [0, N)
+  const int64_t valid = kStripCols - 4 * T;
+  const int64_t strips = ceil_div(N - 1, valid);  // output columns [0, N-1)
   const int64_t chunks = ceil_div(N - 2, R);
-  const int64_t waves = strips * chunks;
-  const int64_t grid = ceil_div(waves, 4);
+  const int64_t grid = ceil_div(strips * chunks, 4);
   if (grid > INT32_MAX || strips > INT32_MAX || chunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
-  jacobi_fused_kernel<R><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, int(strips), int(chunks));
+  jacobi_fused_kernel<R, T><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, int(strips), int(chunks));
   return launch_status();
 }
 
-int fused_step(double* Aout, double* B, const double* Ain, int64_t N, int rows, hipStream_t s) {
+template <int T>
+int fused_rows(double* Aout, double* B, const double* Ain, int64_t N, int rows, hipStream_t s) {
   switch (rows) {
-    case 8: return launch_fused<8>(Aout, B, Ain, N, s);
-    case 16: return launch_fused<16>(Aout, B, Ain, N, s);
-    case 32: return launch_fused<32>(Aout, B, Ain, N, s);
+    case 8: return launch_fused<8, T>(Aout, B, Ain, N, s);
+    case 16: return launch_fused<16, T>(Aout, B, Ain, N, s);
+    case 32: return launch_fused<32, T>(Aout, B, Ain, N, s);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+// `depth` time steps in one launch, reading Ain and writing B and Aout.
+int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows, int depth, hipStream_t s) {
+  switch (depth) {
+    case 1: return fused_rows<1>(Aout, B, Ain, N, rows, s);
+    case 2: return fused_rows<2>(Aout, B, Ain, N, rows, s);
+    case 3: return fused_rows<3>(Aout, B, Ain, N, rows, s);
     default: return RPX_ERR_UNSUPPORTED;
   }
 }
 
 constexpr int kDefaultRows = 16;
 constexpr int kDefaultFusedRows = 16;
+constexpr int kDefaultDepth = 2;  // time steps per fused launch
+constexpr int kMaxDepth = 3;
 
 template <int R>
 int launch_vec(double* dst, const double* src, int64_t N, hipStream_t s) {
@@ -287,6 +303,23 @@ int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N, const rpx_
 
 size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N) { return N > 0 ? size_t(N) * size_t(N) * sizeof(double) : 0; }
 
+int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain, int64_t N, int depth,
+                                  const rpx_tuning* tune, void* stream) {
+  if (N < 0 || !Aout || !B || !Ain || Aout == Ain || Aout == B || B == Ain) return RPX_ERR_INVALID_ARGUMENT;
+  if (N % 2 != 0 || !aligned16(Aout) || !aligned16(B) || !aligned16(Ain)) return RPX_ERR_UNSUPPORTED;
+  if (N < 3) return RPX_SUCCESS;
+  const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
+  if ((rows != 8 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth) return RPX_ERR_UNSUPPORTED;
+  return fused_launch(Aout, B, Ain, N, rows, depth, as_stream(stream));
+}
+
+int rpx_jacobi2d_copy_boundary_f64(double* dst, const double* src, int64_t N, void* stream) {
+  if (N < 0 || (N > 0 && (!dst || !src))) return RPX_ERR_INVALID_ARGUMENT;
+  if (N == 0) return RPX_SUCCESS;
+  boundary_copy_kernel<<<dim3(unsigned(ceil_div(N, 256))), dim3(256), 0, as_stream(stream)>>>(dst, src, N);
+  return launch_status();
+}
+
 int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void* scratch, size_t scratch_bytes,
                            const rpx_tuning* tune, void* stream) {
   if (N < 0 || tsteps < 0 || (N > 0 && (!A || !B)) || A == B) return RPX_ERR_INVALID_ARGUMENT;
@@ -297,26 +330,53 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   if (!scratch || scratch == A || scratch == B) return RPX_ERR_INVALID_ARGUMENT;
   if (scratch_bytes < rpx_jacobi2d_fused_scratch_bytes(N)) return RPX_ERR_SCRATCH_TOO_SMALL;
   const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
-  if (rows != 8 && rows != 16 && rows != 32) return RPX_ERR_UNSUPPORTED;
+  const int depth = tune_or(tune, &rpx_tuning::depth, kDefaultDepth);
+  if ((rows != 8 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth) return RPX_ERR_UNSUPPORTED;
+
+  // Launch plan: depths summing to tsteps, an even number of launches so the
+  // A <-> scratch ping-pong ends in A.  A lone single step runs as the
+  // two in-place sweeps instead.
+  int64_t nfull = tsteps / depth;
+  int rem = int(tsteps % depth);
+  int64_t t_done = 0;
+  int first_split[2] = {0, 0};  // an extra pair of launches taken from one full or remainder launch
+  int64_t count = nfull + (rem > 0 ? 1 : 0);
+  if (count % 2 == 1) {
+    if (rem >= 2) {  // split the remainder
+      first_split[0] = rem - 1;
+      first_split[1] = 1;
+      rem = 0;
+    } else if (nfull >= 1 && depth >= 2) {  // split one full launch
+      first_split[0] = depth - 1;
+      first_split[1] = 1;
+      --nfull;
+    } else {  // one single step left over: two in-place sweeps
+      int rc = sweep(B, A, N, nullptr, s);
+      if (rc == RPX_SUCCESS) rc = sweep(A, B, N, nullptr, s);
+      if (rc != RPX_SUCCESS) return rc;
+      t_done = 1;
+      if (rem == 1) rem = 0;
+      else --nfull;
+    }
+  }
+  if (t_done == tsteps) return RPX_SUCCESS;
   double* Q = reinterpret_cast<double*>(scratch);
-  int64_t t = 0;
-  if (tsteps % 2 == 1) {  // one two-sweep step first, then pairs of fused steps end in A
-    int rc = sweep(B, A, N, nullptr, s);
-    if (rc == RPX_SUCCESS) rc = sweep(A, B, N, nullptr, s);
-    if (rc != RPX_SUCCESS) return rc;
-    t = 1;
-  }
-  if (t < tsteps) {
-    boundary_copy_kernel<<<dim3(unsigned(ceil_div(N, 256))), dim3(256), 0, s>>>(Q, A, N);
-    const int rc = launch_status();
-    if (rc != RPX_SUCCESS) return rc;
-  }
-  for (; t < tsteps; t += 2) {
-    int rc = fused_step(Q, B, A, N, rows, s);
-    if (rc == RPX_SUCCESS) rc = fused_step(A, B, Q, N, rows, s);
-    if (rc != RPX_SUCCESS) return rc;
-  }
-  return RPX_SUCCESS;
+  boundary_copy_kernel<<<dim3(unsigned(ceil_div(N, 256))), dim3(256), 0, s>>>(Q, A, N);
+  int rc = launch_status();
+  double* src = A;
+  double* dst = Q;
+  auto go = [&](int d) {
+    if (rc != RPX_SUCCESS || d == 0) return;
+    rc = fused_launch(dst, B, src, N, rows, d, s);
+    double* t = src;
+    src = dst;
+    dst = t;
+  };
+  go(first_split[0]);
+  go(first_split[1]);
+  for (int64_t i = 0; i < nfull; ++i) go(depth);
+  go(rem);
+  return rc;
 }
 
 int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps, const rpx_tuning* tune, void* stream) {

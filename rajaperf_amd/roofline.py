@@ -96,10 +96,18 @@ def config2(rows, reps):
     sweep = 8 * (N * N - 4) + 8 * (N - 2) * (N - 2)
     ms = _time(lambda: L.rpx_jacobi2d_f64(gA.ptr, gB.ptr, N, 50, None, None), max(1, reps // 2))
     _row(rows, "Polybench_JACOBI_2D(two-sweep, per sweep)", 2, ms / 100, sweep)
-    jsb = L.rpx_jacobi2d_fused_scratch_bytes(N)
-    gQ = B(jsb)
-    ms = _time(lambda: L.rpx_jacobi2d_fused_f64(gA.ptr, gB.ptr, N, 50, gQ.ptr, jsb, None, None), max(1, reps // 2))
-    _row(rows, "Polybench_JACOBI_2D(fused, per time step)", 2, ms / 50, 2 * sweep)
+    # Temporally blocked launches: `depth` time steps (2*depth sweeps) each,
+    # the new A ping-ponging between two grids.
+    gQ = B(L.rpx_jacobi2d_fused_scratch_bytes(N))
+    L.rpx_jacobi2d_copy_boundary_f64(gQ.ptr, gA.ptr, N, None)
+    for depth in (1, 2, 3):
+        grids = [gA.ptr, gQ.ptr]
+
+        def launch(depth=depth, grids=grids):
+            L.rpx_jacobi2d_fused_launch_f64(grids[1], gB.ptr, grids[0], N, depth, None, None)
+            grids.reverse()
+        ms = _time(launch, 4 * reps)
+        _row(rows, f"Polybench_JACOBI_2D(fused depth {depth}, per launch)", 2, ms, 2 * depth * sweep)
 
 
 def config3(rows, reps):

@@ -88,7 +88,7 @@ Real_type KernelBase::initScalar(Real_type seeded_value) {
 }
 
 rpx_tuning KernelBase::defaultTuning() const {
-  rpx_tuning t{run_params.getGPUBlockSize(), run_params.getItemsPerThread(), 0, 0};
+  rpx_tuning t{run_params.getGPUBlockSize(), run_params.getItemsPerThread(), 0, 0, 0};
   return t;
 }
 

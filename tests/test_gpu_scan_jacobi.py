@@ -81,16 +81,16 @@ def test_jacobi_bit_exact(rpx, orc, rng, N, tsteps):
     assert np.array_equal(dB.to_array(), B)
 
 
-@pytest.mark.parametrize("N", [3, 4, 6, 8, 130, 250, 251, 1026, 2050])
-@pytest.mark.parametrize("tsteps", [1, 2, 5])
-@pytest.mark.parametrize("rows", [8, 16, 32])
-def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows):
+@pytest.mark.parametrize("N", [3, 4, 6, 8, 10, 130, 250, 251, 1026, 2050])
+@pytest.mark.parametrize("tsteps", [1, 2, 3, 5, 6])
+@pytest.mark.parametrize("rows,depth", [(8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2), (8, 3), (16, 3)])
+def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows, depth):
     A0 = rng.random(N * N)
     B0 = rng.random(N * N)
     dA, dB = dev(rpx, A0), dev(rpx, B0)
     sb = rpx.lib().rpx_jacobi2d_fused_scratch_bytes(N)
     scr = rpx.DeviceBuffer(sb)
-    rpx.lib().rpx_jacobi2d_fused_f64(dA.ptr, dB.ptr, N, tsteps, scr.ptr, sb, rpx.tuning(0, rows), None)
+    rpx.lib().rpx_jacobi2d_fused_f64(dA.ptr, dB.ptr, N, tsteps, scr.ptr, sb, rpx.tuning(0, rows, depth=depth), None)
     A, B = orc.jacobi2d(A0, B0, N, tsteps)
     assert np.array_equal(dA.to_array(), A)
     assert np.array_equal(dB.to_array(), B)
