@@ -59,8 +59,12 @@ typedef struct rpx_tuning {
                                 fused launch (1..3)                          */
 } rpx_tuning;
 
-#define RPX_TUNE_NONTEMPORAL_STORE 0x1 /* streaming stores bypass the cache */
-#define RPX_TUNE_NAIVE             0x2 /* GEMM: one-thread-per-C comparator */
+#define RPX_TUNE_CACHED            0x1 /* stream/DOT/REDUCE_SUM/SCAN: regular cached
+                                           loads and stores instead of the default
+                                           nontemporal (streaming) ones */
+#define RPX_TUNE_NAIVE             0x2 /* plain FP64 comparators in the suite's Base_HIP
+                                           shape: GEMM one work-item per C element,
+                                           MASS3DPA one element per Q x Q workgroup */
 
 /* ---- library / runtime support ---------------------------------------- *
  * Counterparts of the suite's HIP data utilities
@@ -130,11 +134,15 @@ int rpx_triad_f64(double* a, const double* b, const double* c, double alpha,
                   int64_t n, const rpx_tuning* tune, void* stream);
 
 /* ---- reductions -------------------------------------------------------- *
- * Deterministic: a fixed grid writes one FP64 partial per workgroup into
- * `scratch`; the last workgroup to finish sums the partials in index order.
- * Results are identical from run to run for a fixed (n, tune).
- * `scratch` must be zeroed once before its first use
- * (rpx_reduce_scratch_init); every call leaves it ready for the next. */
+ * Deterministic: each workgroup reduces one fixed chunk of the input
+ * (tune->items_per_thread doubles per work-item, a power of two >= 2;
+ * tune->grid_size, when set, caps the workgroup count by lengthening the
+ * chunks) and writes one FP64 partial into `scratch`; partials are combined
+ * in index order through two ticket levels (groups of 64 workgroups, then
+ * the groups).  Results are identical from run to run for a fixed
+ * (n, tune).  x (and b for DOT) must be 16-byte aligned.  `scratch` must be
+ * zeroed once before its first use (rpx_reduce_scratch_init); every call
+ * leaves it ready for the next. */
 size_t rpx_reduce_scratch_bytes(int64_t n, const rpx_tuning* tune);
 int rpx_reduce_scratch_init(void* scratch, size_t scratch_bytes,
                             void* stream);
@@ -153,8 +161,10 @@ int rpx_reduce_sum_f64(const double* x, int64_t n, double* result,
 /* ---- scan -------------------------------------------------------------- *
  * Exclusive prefix sum y[i] = x[0]+...+x[i-1], y[0] = 0
  * (src/algorithm/SCAN.hpp:27-32, SCAN-Seq.cpp:31-45, SCAN-Hip.cpp:39-326).
- * Single pass with decoupled look-back; tiles take ticketed ids so a tile
- * only waits on tiles that were scheduled before it.  x and y may alias. */
+ * Single pass with decoupled look-back over tiles of 256 x
+ * tune->items_per_thread doubles (16, 32 or 64; default 64); a tile only
+ * waits on tiles with lower workgroup ids, which are dispatched before it.
+ * The scratch size does not depend on the tuning.  x and y may alias. */
 size_t rpx_scan_scratch_bytes(int64_t n, const rpx_tuning* tune);
 /* Zeroes a scan scratch buffer; required once before its first use.  Each
  * call then leaves the buffer ready for the next (epoch-tagged flags). */

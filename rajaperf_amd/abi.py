@@ -28,7 +28,7 @@ RPX_ERR_UNSUPPORTED = -1003
 RPX_ERR_NO_DEVICE = -1004
 RPX_ERR_WAIT_EXPIRED = -1005
 
-RPX_TUNE_NONTEMPORAL_STORE = 0x1
+RPX_TUNE_CACHED = 0x1
 RPX_TUNE_NAIVE = 0x2
 
 

@@ -6,10 +6,15 @@
 // ready flags every rep and spins on predecessors indexed by blockIdx.x.
 //
 // MI355X design:
-//  * a tile is 256 work-items x 16 doubles = 32 KiB, loaded with striped
-//    16-byte loads (each wavefront instruction covers 1 KiB), transposed
-//    through padded LDS to a blocked layout for the per-thread serial scan,
-//    and transposed back for striped 16-byte stores;
+//  * a tile is 256 work-items x IPT doubles (IPT 16, 32 or 64: 32-128 KiB).
+//    Each wavefront owns a contiguous quarter of the tile and reads it as
+//    IPT/2 rows of 1 KiB (one 16-byte load per lane per row); it scans the
+//    rows in order in registers -- lane pair sum, a 64-lane shuffle scan and
+//    a running row carry -- so no LDS staging buffer limits residency, and
+//    the result leaves through the same 16-byte row stores.  Row loads and
+//    stores are nontemporal unless RPX_TUNE_CACHED is set.  Larger tiles
+//    mean fewer look-back hops: 2^28 doubles scan at 4.0, 4.9 and 5.5 TB/s
+//    with IPT 16, 32 and 64 on MI355X;
 //  * the tile id is the workgroup id: each XCD dispatches its workgroups in
 //    id order, so the lowest unfinished tile is always resident and the
 //    look-back always makes progress.  (A ticket counter taken with one
@@ -38,13 +43,9 @@ struct ScanHeader {
 };
 constexpr size_t kHeaderBytes = 256;
 constexpr int kBS = 256;
-constexpr int kIPT = 16;
-constexpr int kTile = kBS * kIPT;
-constexpr int kPadShift = 4;  // one pad double per 16
-constexpr int kLdsDoubles = kTile + (kTile >> kPadShift);
+constexpr int kDefaultIPT = 64;
+constexpr int kMinTile = kBS * 16;  // scratch is sized for the smallest tile
 constexpr unsigned kSpinLimit = 1u << 22;
-
-__device__ __forceinline__ int pad(int e) { return e + (e >> kPadShift); }
 
 // All look-back state (flags and values) moves through agent-scope atomic
 // loads and stores, which gfx950 performs coherently across the XCDs' L2s
@@ -72,11 +73,15 @@ __device__ __forceinline__ void store_value(double* p, double v) {
                      __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <int ROWS, bool NT>  // 16-byte rows per lane: a wavefront scans 128*ROWS doubles
 __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, int64_t n, int64_t ntiles,
                                                    ScanHeader* __restrict__ hdr,
                                                    unsigned long long* __restrict__ flags,
-                                                   double* __restrict__ aggs, double* __restrict__ incls) {
-  __shared__ double tile[kLdsDoubles];
+                                                   double* __restrict__ aggs, double* __restrict__ incls,
+                                                   int vec) {
+  constexpr int kRowLen = 2 * kWave;            // doubles per wavefront row
+  constexpr int kChunk = kRowLen * ROWS;        // doubles per wavefront
+  constexpr int kTileN = kChunk * (kBS / kWave);
   __shared__ double wave_tot[kBS / kWave];
   __shared__ unsigned long long s_epoch;
   __shared__ double s_prefix;
@@ -84,48 +89,46 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
-
   if (tid == 0) s_epoch = __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const int64_t t = blockIdx.x;
-  const unsigned long long E = s_epoch;
-  const int64_t start = t * kTile;
-  const int64_t valid = (n - start < kTile) ? (n - start) : kTile;
 
-  // Striped load -> LDS.
-  const bool full = (valid == kTile) && ((reinterpret_cast<uintptr_t>(x + start) & 15u) == 0);
+  const int64_t t = blockIdx.x;
+  const int64_t base = t * kTileN + int64_t(wave) * kChunk + 2 * lane;
+  const bool full = vec && (t + 1) * kTileN <= n;
+
+  // Each wavefront owns a contiguous chunk, read as ROWS coalesced 1 KiB rows.
+  v2d v[ROWS];
   if (full) {
-    const v2d* xv = reinterpret_cast<const v2d*>(x + start);
-    v2d r[kIPT / 2];
 #pragma unroll
-    for (int k = 0; k < kIPT / 2; ++k) r[k] = xv[k * kBS + tid];
-#pragma unroll
-    for (int k = 0; k < kIPT / 2; ++k) {
-      const int e = 2 * (k * kBS + tid);
-      tile[pad(e)] = r[k].x;
-      tile[pad(e + 1)] = r[k].y;
+    for (int r = 0; r < ROWS; ++r) {
+      const v2d* p = reinterpret_cast<const v2d*>(x + base + r * kRowLen);
+      v[r] = NT ? __builtin_nontemporal_load(p) : *p;
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < kIPT; ++k) {
-      const int e = k * kBS + tid;
-      tile[pad(e)] = (e < valid) ? x[start + e] : 0.0;
+    for (int r = 0; r < ROWS; ++r) {
+      const int64_t i0 = base + r * kRowLen;
+      v[r].x = (i0 < n) ? x[i0] : 0.0;
+      v[r].y = (i0 + 1 < n) ? x[i0 + 1] : 0.0;
     }
   }
-  __syncthreads();
 
-  // Blocked serial scan of kIPT items per work-item.
-  double v[kIPT];
-  double tsum = 0.0;
+  // Exclusive scan inside the wavefront, row by row: lane pair sums, a
+  // 64-lane scan, and a running carry of the row totals.
+  double carry = 0.0;
 #pragma unroll
-  for (int i = 0; i < kIPT; ++i) {
-    v[i] = tile[pad(tid * kIPT + i)];
-    tsum += v[i];
+  for (int r = 0; r < ROWS; ++r) {
+    const double p = v[r].x + v[r].y;
+    const double inc = wave_inclusive_scan(p, lane);
+    double exc = __shfl_up(inc, 1, kWave);
+    if (lane == 0) exc = 0.0;
+    const double row_total = __shfl(inc, kWave - 1, kWave);
+    const double e0 = carry + exc;
+    v[r] = v2d{e0, e0 + v[r].x};
+    carry += row_total;
   }
-  // Work-item totals: wave scan, then across waves.
-  const double incl = wave_inclusive_scan(tsum, lane);
-  if (lane == kWave - 1) wave_tot[wave] = incl;
+  if (lane == 0) wave_tot[wave] = carry;
   __syncthreads();
+  const unsigned long long E = s_epoch;
   double wave_off = 0.0, tile_agg = 0.0;
 #pragma unroll
   for (int w = 0; w < kBS / kWave; ++w) {
@@ -133,7 +136,6 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
     if (w < wave) wave_off += wt;
     tile_agg += wt;
   }
-  const double thread_excl = wave_off + (incl - tsum);
 
   // Publish the aggregate, then look back (wavefront 0).
   if (wave == 0) {
@@ -192,38 +194,23 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
   }
   __syncthreads();
 
-  // Exclusive values: running sum from the tile prefix plus the work-item offset.
-  double run = s_prefix + thread_excl;
+  const double off = s_prefix + wave_off;
+  if (full) {
 #pragma unroll
-  for (int i = 0; i < kIPT; ++i) {
-    const double xi = v[i];
-    v[i] = run;
-    run += xi;
-  }
-  __syncthreads();  // everyone has read its blocked inputs
-#pragma unroll
-  for (int i = 0; i < kIPT; ++i) tile[pad(tid * kIPT + i)] = v[i];
-  __syncthreads();
-
-  const bool full_out = (valid == kTile) && ((reinterpret_cast<uintptr_t>(y + start) & 15u) == 0);
-  if (full_out) {
-    v2d* yv = reinterpret_cast<v2d*>(y + start);
-#pragma unroll
-    for (int k = 0; k < kIPT / 2; ++k) {
-      const int e = 2 * (k * kBS + tid);
-      v2d r;
-      r.x = tile[pad(e)];
-      r.y = tile[pad(e + 1)];
-      yv[k * kBS + tid] = r;
+    for (int r = 0; r < ROWS; ++r) {
+      v2d* p = reinterpret_cast<v2d*>(y + base + r * kRowLen);
+      const v2d o = v2d{off + v[r].x, off + v[r].y};
+      if (NT) __builtin_nontemporal_store(o, p);
+      else *p = o;
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < kIPT; ++k) {
-      const int e = k * kBS + tid;
-      if (e < valid) y[start + e] = tile[pad(e)];
+    for (int r = 0; r < ROWS; ++r) {
+      const int64_t i0 = base + r * kRowLen;
+      if (i0 < n) y[i0] = off + v[r].x;
+      if (i0 + 1 < n) y[i0 + 1] = off + v[r].y;
     }
   }
-
 }
 
 __global__ void zero_kernel(unsigned* p, int64_t nwords) {
@@ -245,16 +232,33 @@ Layout layout_for(int64_t ntiles) {
   return l;
 }
 
-int scan_one(const double* x, double* y, int64_t n, void* scratch, size_t scratch_bytes, hipStream_t s) {
+template <int ROWS>
+void launch_rows(bool nt, unsigned grid, const double* x, double* y, int64_t n, int64_t ntiles, ScanHeader* hdr,
+                 unsigned long long* flags, double* aggs, double* incls, int vec, hipStream_t s) {
+  if (nt)
+    scan_kernel<ROWS, true><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec);
+  else
+    scan_kernel<ROWS, false><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec);
+}
+
+int scan_one(const double* x, double* y, int64_t n, int ipt, bool nt, void* scratch, size_t scratch_bytes,
+             hipStream_t s) {
   if (n == 0) return RPX_SUCCESS;
-  const int64_t ntiles = ceil_div(n, kTile);
+  if (ipt != 16 && ipt != 32 && ipt != 64) return RPX_ERR_INVALID_ARGUMENT;
+  const int64_t ntiles = ceil_div(n, int64_t(kBS) * ipt);
   if (ntiles > INT32_MAX) return RPX_ERR_UNSUPPORTED;
   const Layout l = layout_for(ntiles);
   if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
   char* base = reinterpret_cast<char*>(scratch);
-  scan_kernel<<<dim3(unsigned(ntiles)), dim3(kBS), 0, s>>>(
-      x, y, n, ntiles, reinterpret_cast<ScanHeader*>(base), reinterpret_cast<unsigned long long*>(base + l.flags_off),
-      reinterpret_cast<double*>(base + l.aggs_off), reinterpret_cast<double*>(base + l.incls_off));
+  auto* hdr = reinterpret_cast<ScanHeader*>(base);
+  auto* flags = reinterpret_cast<unsigned long long*>(base + l.flags_off);
+  auto* aggs = reinterpret_cast<double*>(base + l.aggs_off);
+  auto* incls = reinterpret_cast<double*>(base + l.incls_off);
+  const int vec = aligned16(x) && aligned16(y);
+  const unsigned grid = unsigned(ntiles);
+  if (ipt == 16) launch_rows<8>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, s);
+  else if (ipt == 32) launch_rows<16>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, s);
+  else launch_rows<32>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, s);
   return launch_status();
 }
 
@@ -264,13 +268,15 @@ extern "C" {
 
 size_t rpx_scan_scratch_bytes(int64_t n, const rpx_tuning* /*tune*/) {
   if (n < 0) return 0;
-  return layout_for(ceil_div(n > 0 ? n : 1, kTile)).total;
+  return layout_for(ceil_div(n > 0 ? n : 1, kMinTile)).total;
 }
 
 int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n, void* scratch, size_t scratch_bytes,
-                           const rpx_tuning* /*tune*/, void* stream) {
+                           const rpx_tuning* tune, void* stream) {
   if (n < 0 || (n > 0 && (!x || !y || !scratch))) return RPX_ERR_INVALID_ARGUMENT;
-  return scan_one(x, y, n, scratch, scratch_bytes, as_stream(stream));
+  const int ipt = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultIPT);
+  const bool nt = (tune_flags(tune) & RPX_TUNE_CACHED) == 0;
+  return scan_one(x, y, n, ipt, nt, scratch, scratch_bytes, as_stream(stream));
 }
 
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host, int64_t nseg) {
@@ -293,7 +299,7 @@ int rpx_scan_exclusive_segmented_f64(const double* x, double* y, const int64_t* 
   // which is legal because consecutive launches on one stream are ordered.
   for (int64_t s = 0; s < nseg; ++s) {
     const int64_t b = offsets_host[s], len = offsets_host[s + 1] - b;
-    const int rc = scan_one(x + b, y + b, len, scratch, scratch_bytes, as_stream(stream));
+    const int rc = scan_one(x + b, y + b, len, kDefaultIPT, true, scratch, scratch_bytes, as_stream(stream));
     if (rc != RPX_SUCCESS) return rc;
   }
   return RPX_SUCCESS;

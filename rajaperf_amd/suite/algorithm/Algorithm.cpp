@@ -69,6 +69,11 @@ SCAN::SCAN(const RunParams& params) : KernelBase(Algorithm_SCAN, params) {
   setComplexity(Complexity::N);
   setVariantDefined(MI355X_Native);
 }
+void SCAN::setMI355XTuningDefinitions(VariantID vid) {
+  addVariantTuningName(vid, "default");  // 64 doubles per work-item
+  addVariantTuningName(vid, "items_16");
+  addVariantTuningName(vid, "items_32");
+}
 void SCAN::setUp(VariantID, size_t) {
   allocAndInitDataRandValue(m_x, getActualProblemSize(), -1.0, 1.0);
   allocAndInitDataConst(m_y, getActualProblemSize(), 0.0);
@@ -80,14 +85,17 @@ void SCAN::tearDown(VariantID, size_t) {
   deallocData(m_x);
   deallocData(m_y);
 }
-void SCAN::runMI355XVariant(VariantID, size_t) {
+void SCAN::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  const size_t sb = rpx_scan_scratch_bytes(n, nullptr);
+  rpx_tuning tune = defaultTuning();
+  if (tune_idx == 1) tune.items_per_thread = 16;
+  if (tune_idx == 2) tune.items_per_thread = 32;
+  const size_t sb = rpx_scan_scratch_bytes(n, &tune);
   void* scratch = detail::deviceAlloc(sb);
   rpxCheck(rpx_scan_scratch_init(scratch, sb, nullptr), "scan scratch init");
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, nullptr, nullptr), "SCAN");
+    rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, &tune, nullptr), "SCAN");
   stopTimer();
   rpxCheck(rpx_scan_scratch_status(scratch), "SCAN look-back");
   detail::deviceFree(scratch);

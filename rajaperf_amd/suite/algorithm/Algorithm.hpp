@@ -24,6 +24,7 @@ class REDUCE_SUM : public KernelBase {
 class SCAN : public KernelBase {
  public:
   explicit SCAN(const RunParams& params);
+  void setMI355XTuningDefinitions(VariantID vid) override;
   void setUp(VariantID vid, size_t tune_idx) override;
   void updateChecksum(VariantID vid, size_t tune_idx) override;
   void tearDown(VariantID vid, size_t tune_idx) override;

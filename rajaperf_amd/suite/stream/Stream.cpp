@@ -21,7 +21,7 @@ struct StreamTune {
 };
 const StreamTune kStreamTunes[] = {
     {"default", 0, 0, 0},        {"block_128", 128, 0, 0}, {"block_512", 512, 0, 0},
-    {"block_1024", 1024, 0, 0},  {"items_16", 0, 16, 0},   {"nontemporal", 0, 0, RPX_TUNE_NONTEMPORAL_STORE},
+    {"block_1024", 1024, 0, 0},  {"items_16", 0, 16, 0},   {"cached", 0, 0, RPX_TUNE_CACHED},
 };
 }  // namespace
 

@@ -117,6 +117,26 @@ def test_dot_and_reduce_sum(rpx, orc, rng, n):
     assert abs(got - ref) <= 4e-16 * max(1, n) ** 0.5 * np.sum(np.abs(x)) + 1e-300
 
 
+@pytest.mark.parametrize("n,items,flags", [
+    (64 * 8192, 32, 0),          # exactly one full group of 64 workgroups
+    (3 * 64 * 8192, 32, 0),      # three full groups
+    (3 * 64 * 8192 + 3, 32, 1),  # a fourth group with one workgroup, odd tail
+    (1_000_001, 2, 0), (1_000_001, 4, 1), (1_000_001, 8, 0), (1_000_001, 64, 0),
+])
+def test_reduction_ticket_levels_exact(rpx, rng, n, items, flags):
+    a = rng.integers(-1000, 1000, n).astype(np.float64)
+    b = rng.integers(-1000, 1000, n).astype(np.float64)
+    da, db = dev(rpx, a), dev(rpx, b)
+    t = rpx.tuning(0, items, 0, flags)
+    scratch, sb, res = _reduce(rpx, None, n=n, tune=t)
+    L = rpx.lib()
+    for _ in range(2):  # tickets of both levels are reset for the next call
+        L.rpx_dot_f64(da.ptr, db.ptr, n, res.ptr, 0, scratch.ptr, sb, t, None)
+        assert res.to_array()[0] == float(np.dot(a.astype(np.int64), b.astype(np.int64)))
+        L.rpx_reduce_sum_f64(db.ptr, n, res.ptr, scratch.ptr, sb, t, None)
+        assert res.to_array()[0] == float(b.astype(np.int64).sum())
+
+
 def test_reductions_exact_on_integers_and_deterministic(rpx, rng):
     n = 5_000_011
     a = rng.integers(-1000, 1000, n).astype(np.float64)
