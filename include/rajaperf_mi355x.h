@@ -65,6 +65,9 @@ typedef struct rpx_tuning {
 #define RPX_TUNE_NAIVE             0x2 /* plain FP64 comparators in the suite's Base_HIP
                                            shape: GEMM one work-item per C element,
                                            MASS3DPA one element per Q x Q workgroup */
+#define RPX_TUNE_JACOBI_SKIP_B     0x4 /* rpx_jacobi2d_fused_launch_f64: leave B's
+                                           interior unwritten, for a launch whose B a
+                                           later launch rewrites before it is read */
 
 /* ---- library / runtime support ---------------------------------------- *
  * Counterparts of the suite's HIP data utilities
@@ -209,7 +212,7 @@ int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N,
 int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
                      const rpx_tuning* tune, void* stream);
 /* Same result with temporal blocking: each launch runs `depth` whole time
- * steps (2*depth sweeps, tune->depth, default 2) on register-resident tiles,
+ * steps (2*depth sweeps, tune->depth, default 3) on register-resident tiles,
  * reading A once and writing only the final B and the new A, the new A going
  * to `scratch` (an N x N grid) and back, so no launch overwrites values a
  * neighbouring tile still reads.  Ends with the result in A and B, bit-equal
@@ -217,10 +220,14 @@ int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
  * pointers that are not 16-byte aligned. */
 size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
 /* The building block of rpx_jacobi2d_fused_f64: one launch advancing `depth`
- * time steps (1..3) from Ain into Aout, writing the final B.  Requires even
- * N, 16-byte aligned grids, Aout != Ain, and Aout's boundary equal to Ain's
- * (copy it once with rpx_jacobi2d_copy_boundary_f64); callers ping-pong Ain
- * and Aout between launches. */
+ * time steps (1..3) from Ain into Aout, writing the final B (unless
+ * tune->flags has RPX_TUNE_JACOBI_SKIP_B: in a run of launches only the last
+ * one needs to store B).  Requires even N, 16-byte aligned grids,
+ * Aout != Ain, and Aout's boundary equal to Ain's (copy it once with
+ * rpx_jacobi2d_copy_boundary_f64; the launch may rewrite boundary columns
+ * with those same values); callers ping-pong Ain and Aout between launches.
+ * tune->items_per_thread: output rows per wavefront tile, 4, 8 (default),
+ * 12, 16 or 32. */
 int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain,
                                   int64_t N, int depth,
                                   const rpx_tuning* tune, void* stream);

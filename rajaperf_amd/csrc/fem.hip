@@ -174,6 +174,78 @@ __global__ __launch_bounds__(kBS) void mass3dpa_kernel(const double* __restrict_
   }
 }
 
+// The plain FP64 comparator in the shape of the suite's Base_HIP variant
+// (MASS3DPA-Hip.cpp:25-117): one element per workgroup of Q x Q work-items,
+// two Q^3 LDS buffers, a barrier between stages.  Same sums in the same order
+// as mass3dpa_kernel, so both are bit-identical to Base_Seq.
+template <int D, int Q>
+__global__ __launch_bounds__(Q* Q) void mass3dpa_naive_kernel(const double* __restrict__ Bg,
+                                                               const double* __restrict__ Btg,
+                                                               const double* __restrict__ Dg,
+                                                               const double* __restrict__ Xg,
+                                                               double* __restrict__ Yg) {
+  constexpr int D2 = D * D, D3 = D * D * D, Q3 = Q * Q * Q;
+  __shared__ double sB[Q * D];   // sB[q*D + d]  = B(q,d)
+  __shared__ double sBt[D * Q];  // sBt[d*Q + q] = Bt'(d,q)
+  __shared__ double s0[Q3];
+  __shared__ double s1[Q3];
+  const int tx = threadIdx.x % Q, ty = threadIdx.x / Q;
+  const int64_t e = blockIdx.x;
+  for (int t = threadIdx.x; t < Q * D; t += Q * Q) {
+    sB[t] = Bg[t / D + Q * (t % D)];
+    sBt[t] = Btg[t % Q + D * (t / Q)];
+  }
+  // X[dz][dy][dx] -> s0
+  if (tx < D && ty < D)
+    for (int dz = 0; dz < D; ++dz) s0[(dz * D + ty) * D + tx] = Xg[e * D3 + dz * D2 + ty * D + tx];
+  __syncthreads();
+  // x: s1[dz][dy][qx]   (tx = qx, ty = dy)
+  if (ty < D)
+    for (int dz = 0; dz < D; ++dz) {
+      double u = 0.0;
+      for (int dx = 0; dx < D; ++dx) u += s0[(dz * D + ty) * D + dx] * sB[tx * D + dx];
+      s1[(dz * D + ty) * Q + tx] = u;
+    }
+  __syncthreads();
+  // y: s0[dz][qy][qx]   (tx = qx, ty = qy)
+  for (int dz = 0; dz < D; ++dz) {
+    double u = 0.0;
+    for (int dy = 0; dy < D; ++dy) u += s1[(dz * D + dy) * Q + tx] * sB[ty * D + dy];
+    s0[(dz * Q + ty) * Q + tx] = u;
+  }
+  __syncthreads();
+  // z and D: s1[qz][qy][qx]
+  for (int qz = 0; qz < Q; ++qz) {
+    double u = 0.0;
+    for (int dz = 0; dz < D; ++dz) u += s0[(dz * Q + ty) * Q + tx] * sB[qz * D + dz];
+    s1[(qz * Q + ty) * Q + tx] = u * Dg[e * Q3 + (qz * Q + ty) * Q + tx];
+  }
+  __syncthreads();
+  // backward x: s0[qz][qy][dx]   (tx = dx, ty = qy)
+  if (tx < D)
+    for (int qz = 0; qz < Q; ++qz) {
+      double u = 0.0;
+      for (int qx = 0; qx < Q; ++qx) u += s1[(qz * Q + ty) * Q + qx] * sBt[tx * Q + qx];
+      s0[(qz * Q + ty) * D + tx] = u;
+    }
+  __syncthreads();
+  // backward y: s1[qz][dy][dx]   (tx = dx, ty = dy)
+  if (tx < D && ty < D)
+    for (int qz = 0; qz < Q; ++qz) {
+      double u = 0.0;
+      for (int qy = 0; qy < Q; ++qy) u += s0[(qz * Q + qy) * D + tx] * sBt[ty * Q + qy];
+      s1[(qz * D + ty) * D + tx] = u;
+    }
+  __syncthreads();
+  // backward z: Y[dz][dy][dx] +=
+  if (tx < D && ty < D)
+    for (int dz = 0; dz < D; ++dz) {
+      double u = 0.0;
+      for (int qz = 0; qz < Q; ++qz) u += s1[(qz * D + ty) * D + tx] * sBt[dz * Q + qz];
+      Yg[e * D3 + dz * D2 + ty * D + tx] += u;
+    }
+}
+
 // ---- DIFFUSION3DPA ----------------------------------------------------------
 
 // Effective basis tables of the suite's half-storage scheme
@@ -414,13 +486,26 @@ int launch_mass(const double* B, const double* Bt, const double* Dq, const doubl
   return launch_status();
 }
 
+template <int D, int Q>
+int launch_mass_naive(const double* B, const double* Bt, const double* Dq, const double* X, double* Y, int64_t NE,
+                      hipStream_t s) {
+  if (NE > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  mass3dpa_naive_kernel<D, Q><<<dim3(unsigned(NE)), dim3(Q * Q), 0, s>>>(B, Bt, Dq, X, Y);
+  return launch_status();
+}
+
 template <int D>
-int mass_q(int q1d, const double* B, const double* Bt, const double* Dq, const double* X, double* Y, int64_t NE,
-           hipStream_t s) {
-  if (q1d == D) return launch_mass<D, D>(B, Bt, Dq, X, Y, NE, s);
-  if (q1d == D + 1) return launch_mass<D, D + 1>(B, Bt, Dq, X, Y, NE, s);
+int mass_q(int q1d, bool naive, const double* B, const double* Bt, const double* Dq, const double* X, double* Y,
+           int64_t NE, hipStream_t s) {
+  if (q1d == D)
+    return naive ? launch_mass_naive<D, D>(B, Bt, Dq, X, Y, NE, s) : launch_mass<D, D>(B, Bt, Dq, X, Y, NE, s);
+  if (q1d == D + 1)
+    return naive ? launch_mass_naive<D, D + 1>(B, Bt, Dq, X, Y, NE, s)
+                 : launch_mass<D, D + 1>(B, Bt, Dq, X, Y, NE, s);
   if constexpr (D + 2 <= 9) {
-    if (q1d == D + 2) return launch_mass<D, D + 2>(B, Bt, Dq, X, Y, NE, s);
+    if (q1d == D + 2)
+      return naive ? launch_mass_naive<D, D + 2>(B, Bt, Dq, X, Y, NE, s)
+                   : launch_mass<D, D + 2>(B, Bt, Dq, X, Y, NE, s);
   }
   return RPX_ERR_UNSUPPORTED;
 }
@@ -456,20 +541,21 @@ int diff_d(int d1d, const double* B, const double* G, const double* Dq, const do
 extern "C" {
 
 int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D, const double* X, double* Y, int64_t NE,
-                     int d1d, int q1d, const rpx_tuning* /*tune*/, void* stream) {
+                     int d1d, int q1d, const rpx_tuning* tune, void* stream) {
   if (NE < 0) return RPX_ERR_INVALID_ARGUMENT;
   if (NE == 0) return RPX_SUCCESS;
   if (!B || !Bt || !D || !X || !Y) return RPX_ERR_INVALID_ARGUMENT;
   hipStream_t s = as_stream(stream);
+  const bool naive = (tune_flags(tune) & RPX_TUNE_NAIVE) != 0;
   switch (d1d) {
-    case 1: return mass_q<1>(q1d, B, Bt, D, X, Y, NE, s);
-    case 2: return mass_q<2>(q1d, B, Bt, D, X, Y, NE, s);
-    case 3: return mass_q<3>(q1d, B, Bt, D, X, Y, NE, s);
-    case 4: return mass_q<4>(q1d, B, Bt, D, X, Y, NE, s);
-    case 5: return mass_q<5>(q1d, B, Bt, D, X, Y, NE, s);
-    case 6: return mass_q<6>(q1d, B, Bt, D, X, Y, NE, s);
-    case 7: return mass_q<7>(q1d, B, Bt, D, X, Y, NE, s);
-    case 8: return mass_q<8>(q1d, B, Bt, D, X, Y, NE, s);
+    case 1: return mass_q<1>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 2: return mass_q<2>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 3: return mass_q<3>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 4: return mass_q<4>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 5: return mass_q<5>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 6: return mass_q<6>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 7: return mass_q<7>(q1d, naive, B, Bt, D, X, Y, NE, s);
+    case 8: return mass_q<8>(q1d, naive, B, Bt, D, X, Y, NE, s);
     default: return RPX_ERR_UNSUPPORTED;
   }
 }

@@ -18,6 +18,7 @@
 // bit-identical to Base_Seq.
 #include "rpx_common.hpp"
 
+
 using namespace rpx;
 
 namespace {
@@ -113,6 +114,19 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
   }
 }
 
+// lane l receives v from lane l-1 (lane 0 keeps its own value).
+__device__ __forceinline__ double from_lower_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// lane l receives v from lane l+1 (lane 63 keeps its own value).
+__device__ __forceinline__ double from_upper_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 // ---- fused time steps: T steps (2T sweeps) per launch -----------------------
 //
 // One wavefront owns a strip of 128 columns (its lanes hold column pairs) and
@@ -131,7 +145,7 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
 // another one has already replaced.
 constexpr int kStripCols = 128;
 
-template <int R, int T>
+template <int R, int T, bool WRITE_B>
 __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ Aout, double* __restrict__ B,
                                                            const double* __restrict__ Ain, int64_t N, int strips,
                                                            int chunks) {
@@ -176,8 +190,8 @@ __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ 
       const v2d up = (L == 1) ? a[q - 2] : win[L - 2][0];
       const v2d c = (L == 1) ? a[q - 1] : win[L - 2][1];
       const v2d dn = (L == 1) ? a[q] : win[L - 2][2];
-      const double w = __shfl_up(c.y, 1, kWave);
-      const double e = __shfl_down(c.x, 1, kWave);
+      const double w = from_lower_lane(c.y);
+      const double e = from_upper_lane(c.x);
       v2d r;
       r.x = point(c.x, w, c.y, dn.x, up.x);
       r.y = point(c.y, c.x, e, dn.y, up.y);
@@ -203,7 +217,7 @@ __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ 
       win[L - 1][0] = win[L - 1][1];
       win[L - 1][1] = win[L - 1][2];
       win[L - 1][2] = r;
-      if ((L == H - 1 || L == H) && g >= r0 && g < rend) {
+      if ((L == H || (WRITE_B && L == H - 1)) && g >= r0 && g < rend) {
         double* out = (L == H ? Aout : B) + g * N + jl;
         if (x_out && y_out) {
           *reinterpret_cast<v2d*>(out) = r;
@@ -229,39 +243,48 @@ __global__ __launch_bounds__(256) void boundary_copy_kernel(double* __restrict__
 }
 
 template <int R, int T>
-int launch_fused(double* Aout, double* B, const double* Ain, int64_t N, hipStream_t s) {
+int launch_fused(double* Aout, double* B, const double* Ain, int64_t N, bool write_b, hipStream_t s) {
   const int64_t valid = kStripCols - 4 * T;
   const int64_t strips = ceil_div(N - 1, valid);  // output columns [0, N-1)
   const int64_t chunks = ceil_div(N - 2, R);
   const int64_t grid = ceil_div(strips * chunks, 4);
   if (grid > INT32_MAX || strips > INT32_MAX || chunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
-  jacobi_fused_kernel<R, T><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, int(strips), int(chunks));
+  if (write_b)
+    jacobi_fused_kernel<R, T, true><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, int(strips),
+                                                                               int(chunks));
+  else
+    jacobi_fused_kernel<R, T, false><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, int(strips),
+                                                                                int(chunks));
   return launch_status();
 }
 
 template <int T>
-int fused_rows(double* Aout, double* B, const double* Ain, int64_t N, int rows, hipStream_t s) {
+int fused_rows(double* Aout, double* B, const double* Ain, int64_t N, int rows, bool write_b, hipStream_t s) {
   switch (rows) {
-    case 8: return launch_fused<8, T>(Aout, B, Ain, N, s);
-    case 16: return launch_fused<16, T>(Aout, B, Ain, N, s);
-    case 32: return launch_fused<32, T>(Aout, B, Ain, N, s);
+    case 4: return launch_fused<4, T>(Aout, B, Ain, N, write_b, s);
+    case 8: return launch_fused<8, T>(Aout, B, Ain, N, write_b, s);
+    case 12: return launch_fused<12, T>(Aout, B, Ain, N, write_b, s);
+    case 16: return launch_fused<16, T>(Aout, B, Ain, N, write_b, s);
+    case 32: return launch_fused<32, T>(Aout, B, Ain, N, write_b, s);
     default: return RPX_ERR_UNSUPPORTED;
   }
 }
 
 // `depth` time steps in one launch, reading Ain and writing B and Aout.
-int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows, int depth, hipStream_t s) {
+// write_b = false leaves B's interior unwritten.
+int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows, int depth, bool write_b,
+                 hipStream_t s) {
   switch (depth) {
-    case 1: return fused_rows<1>(Aout, B, Ain, N, rows, s);
-    case 2: return fused_rows<2>(Aout, B, Ain, N, rows, s);
-    case 3: return fused_rows<3>(Aout, B, Ain, N, rows, s);
+    case 1: return fused_rows<1>(Aout, B, Ain, N, rows, write_b, s);
+    case 2: return fused_rows<2>(Aout, B, Ain, N, rows, write_b, s);
+    case 3: return fused_rows<3>(Aout, B, Ain, N, rows, write_b, s);
     default: return RPX_ERR_UNSUPPORTED;
   }
 }
 
 constexpr int kDefaultRows = 16;
-constexpr int kDefaultFusedRows = 16;
-constexpr int kDefaultDepth = 2;  // time steps per fused launch
+constexpr int kDefaultFusedRows = 8;
+constexpr int kDefaultDepth = 3;  // time steps per fused launch
 constexpr int kMaxDepth = 3;
 
 template <int R>
@@ -309,8 +332,10 @@ int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain, in
   if (N % 2 != 0 || !aligned16(Aout) || !aligned16(B) || !aligned16(Ain)) return RPX_ERR_UNSUPPORTED;
   if (N < 3) return RPX_SUCCESS;
   const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
-  if ((rows != 8 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth) return RPX_ERR_UNSUPPORTED;
-  return fused_launch(Aout, B, Ain, N, rows, depth, as_stream(stream));
+  if ((rows != 4 && rows != 8 && rows != 12 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth)
+    return RPX_ERR_UNSUPPORTED;
+  const bool write_b = (tune_flags(tune) & RPX_TUNE_JACOBI_SKIP_B) == 0;
+  return fused_launch(Aout, B, Ain, N, rows, depth, write_b, as_stream(stream));
 }
 
 int rpx_jacobi2d_copy_boundary_f64(double* dst, const double* src, int64_t N, void* stream) {
@@ -331,7 +356,8 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   if (scratch_bytes < rpx_jacobi2d_fused_scratch_bytes(N)) return RPX_ERR_SCRATCH_TOO_SMALL;
   const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
   const int depth = tune_or(tune, &rpx_tuning::depth, kDefaultDepth);
-  if ((rows != 8 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth) return RPX_ERR_UNSUPPORTED;
+  if ((rows != 4 && rows != 8 && rows != 12 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth)
+    return RPX_ERR_UNSUPPORTED;
 
   // Launch plan: depths summing to tsteps, an even number of launches so the
   // A <-> scratch ping-pong ends in A.  A lone single step runs as the
@@ -365,9 +391,13 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   int rc = launch_status();
   double* src = A;
   double* dst = Q;
+  // Only the last launch stores B: each launch's B is overwritten by the
+  // next before anything reads it (a launch reads only B's boundary, which
+  // never changes).
+  int64_t left = (first_split[0] > 0) + (first_split[1] > 0) + nfull + (rem > 0);
   auto go = [&](int d) {
     if (rc != RPX_SUCCESS || d == 0) return;
-    rc = fused_launch(dst, B, src, N, rows, d, s);
+    rc = fused_launch(dst, B, src, N, rows, d, --left == 0, s);
     double* t = src;
     src = dst;
     dst = t;

@@ -76,6 +76,9 @@ def config2(rows, reps):
     _row(rows, "Stream_ADD", 2, _time(lambda: L.rpx_add_f64(a.ptr, b.ptr, c.ptr, n, None, None), reps), 24 * n)
     _row(rows, "Stream_TRIAD", 2, _time(lambda: L.rpx_triad_f64(a.ptr, b.ptr, c.ptr, 0.75, n, None, None), reps),
          24 * n)
+    cached = abi.tuning(0, 0, 0, abi.RPX_TUNE_CACHED)
+    _row(rows, "Stream_TRIAD(cached accesses)", 2,
+         _time(lambda: L.rpx_triad_f64(a.ptr, b.ptr, c.ptr, 0.75, n, cached, None), reps), 24 * n)
     sb = L.rpx_reduce_scratch_bytes(n, None)
     scr, res = B(sb), B(16)
     L.rpx_reduce_scratch_init(scr.ptr, sb, None)
@@ -83,11 +86,19 @@ def config2(rows, reps):
          _time(lambda: L.rpx_dot_f64(b.ptr, c.ptr, n, res.ptr, 0, scr.ptr, sb, None, None), reps), 16 * n + 16)
     _row(rows, "Algorithm_REDUCE_SUM", 2,
          _time(lambda: L.rpx_reduce_sum_f64(b.ptr, n, res.ptr, scr.ptr, sb, None, None), reps), 8 * n + 16)
+    _row(rows, "Stream_DOT(cached accesses)", 2,
+         _time(lambda: L.rpx_dot_f64(b.ptr, c.ptr, n, res.ptr, 0, scr.ptr, sb, cached, None), reps), 16 * n + 16)
     ssb = L.rpx_scan_scratch_bytes(n, None)
     sscr = B(ssb)
     L.rpx_scan_scratch_init(sscr.ptr, ssb, None)
-    _row(rows, "Algorithm_SCAN", 2,
-         _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, None, None), reps), 16 * n)
+    for ipt in (16, 32, 64):
+        t = abi.tuning(0, ipt)
+        _row(rows, f"Algorithm_SCAN(ipt {ipt})", 2,
+             _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, t, None), reps), 16 * n)
+    _row(rows, "Algorithm_SCAN(ipt 64, cached accesses)", 2,
+         _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, cached, None), reps), 16 * n)
+    if L.rpx_scan_scratch_status(sscr.ptr) != 0:
+        raise RuntimeError("scan look-back wait expired")
     del a, b, c
     N = 8192
     gA, gB = B(8 * N * N), B(8 * N * N)
@@ -100,14 +111,27 @@ def config2(rows, reps):
     # the new A ping-ponging between two grids.
     gQ = B(L.rpx_jacobi2d_fused_scratch_bytes(N))
     L.rpx_jacobi2d_copy_boundary_f64(gQ.ptr, gA.ptr, N, None)
-    for depth in (1, 2, 3):
+    # Per launch: `depth` time steps on tiles of `rows` output rows per
+    # wavefront.  "B skipped": the launches of a run other than the last
+    # store no B (each run's B is overwritten before it is read).
+    skip = abi.RPX_TUNE_JACOBI_SKIP_B
+    for depth, rows_t, flags in ((1, 8, 0), (2, 8, 0), (3, 8, 0), (1, 8, skip), (2, 4, skip), (2, 8, skip),
+                                 (2, 12, skip), (3, 4, skip), (3, 8, skip), (3, 12, skip)):
         grids = [gA.ptr, gQ.ptr]
+        t = abi.tuning(0, rows_t, 0, flags)
 
-        def launch(depth=depth, grids=grids):
-            L.rpx_jacobi2d_fused_launch_f64(grids[1], gB.ptr, grids[0], N, depth, None, None)
+        def launch(depth=depth, grids=grids, t=t):
+            L.rpx_jacobi2d_fused_launch_f64(grids[1], gB.ptr, grids[0], N, depth, t, None)
             grids.reverse()
         ms = _time(launch, 4 * reps)
-        _row(rows, f"Polybench_JACOBI_2D(fused depth {depth}, per launch)", 2, ms, 2 * depth * sweep)
+        kind = f"rows {rows_t}" + (", B skipped" if flags else "")
+        _row(rows, f"Polybench_JACOBI_2D(fused depth {depth}, {kind}, per launch)", 2, ms, 2 * depth * sweep)
+    sb = L.rpx_jacobi2d_fused_scratch_bytes(N)
+    for depth, rows_t in ((2, 8), (3, 8), (3, 4)):
+        t = abi.tuning(0, rows_t, 0, 0, depth)
+        ms = _time(lambda: L.rpx_jacobi2d_fused_f64(gA.ptr, gB.ptr, N, 50, gQ.ptr, sb, t, None), max(1, reps // 2))
+        _row(rows, f"Polybench_JACOBI_2D(50 time steps at depth {depth}, rows {rows_t}, per time step)", 2,
+             ms / 50, 2 * sweep)
 
 
 def config3(rows, reps):
@@ -127,6 +151,9 @@ def config3(rows, reps):
                   + 2 * Q * D ** 3 + D ** 3)
     ms = _time(lambda: L.rpx_mass3dpa_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Y.ptr, NE, D, Q, None, None), reps)
     _row(rows, "Apps_MASS3DPA", 3, ms, nbytes, flops)
+    naive = abi.tuning(flags=abi.RPX_TUNE_NAIVE)
+    ms_naive = _time(lambda: L.rpx_mass3dpa_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Y.ptr, NE, D, Q, naive, None), reps)
+    _row(rows, "Apps_MASS3DPA(plain FP64, one element per Q x Q workgroup)", 3, ms_naive, nbytes, flops)
     del Dm
     Dd = B(8 * NE * 6 * Q ** 3)
     L.rpx_fill_sym_tensor_f64(Dd.ptr, NE, Q ** 3, 6, SEED, 2, None)

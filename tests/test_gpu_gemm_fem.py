@@ -47,8 +47,9 @@ def test_gemm_identity_exact(rpx, rng):
 MASS_ORDERS = [(d, q) for d in range(1, 9) for q in (d, d + 1, d + 2) if q <= 9]
 
 
+@pytest.mark.parametrize("naive", [False, True])
 @pytest.mark.parametrize("d1d,q1d", MASS_ORDERS)
-def test_mass3dpa_bit_exact(rpx, orc, rng, d1d, q1d):
+def test_mass3dpa_bit_exact(rpx, orc, rng, d1d, q1d, naive):
     NE = 67
     B = rng.uniform(-1, 1, q1d * d1d)
     Bt = rng.uniform(-1, 1, q1d * d1d)
@@ -56,8 +57,9 @@ def test_mass3dpa_bit_exact(rpx, orc, rng, d1d, q1d):
     X = rng.uniform(-1, 1, NE * d1d ** 3)
     Y0 = rng.uniform(-1, 1, NE * d1d ** 3)
     bufs = [dev(rpx, a) for a in (B, Bt, D, X, Y0)]
+    t = rpx.tuning(flags=rpx.RPX_TUNE_NAIVE) if naive else None
     for _ in range(2):  # Y accumulates over reps like the suite's Y +=
-        rpx.lib().rpx_mass3dpa_f64(*[b.ptr for b in bufs], NE, d1d, q1d, None, None)
+        rpx.lib().rpx_mass3dpa_f64(*[b.ptr for b in bufs], NE, d1d, q1d, t, None)
     ref = orc.mass3dpa(B, Bt, D, X, orc.mass3dpa(B, Bt, D, X, Y0, NE, d1d, q1d), NE, d1d, q1d)
     assert np.array_equal(bufs[4].to_array(), ref)
 

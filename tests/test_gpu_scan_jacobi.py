@@ -19,13 +19,14 @@ def scan_scratch(rpx, n):
     return s, sb
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 17, 4095, 4096, 4097, 12289, 1_000_000, 3_000_017])
-def test_scan_matches_oracle(rpx, orc, rng, n):
+@pytest.mark.parametrize("ipt", [16, 32, 64])
+@pytest.mark.parametrize("n", [0, 1, 2, 17, 4095, 4096, 4097, 8192, 12289, 1_000_000, 3_000_017])
+def test_scan_matches_oracle(rpx, orc, rng, n, ipt):
     x = rng.uniform(-1, 1, n)
     dx = dev(rpx, x)
     y = rpx.DeviceBuffer(8 * max(n, 1))
     s, sb = scan_scratch(rpx, n)
-    rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, None, None)
+    rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(0, ipt), None)
     got = y.to_array(count=n)
     ref = orc.scan(x)
     if n:
@@ -41,11 +42,25 @@ def test_scan_exact_on_integers_large(rpx, rng):
     dx = dev(rpx, x)
     y = rpx.DeviceBuffer(8 * n)
     s, sb = scan_scratch(rpx, n)
-    for _ in range(2):  # the scratch is reusable without re-initialisation
-        rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, None, None)
+    for ipt in (16, 32, 64, 16):  # the scratch is reusable without re-initialisation
+        rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(0, ipt), None)
         got = y.to_array()
         assert got[0] == 0.0
         assert np.array_equal(got[1:], np.cumsum(x)[:-1])
+    assert rpx.lib().rpx_scan_scratch_status(s.ptr) == 0
+
+
+def test_scan_unaligned_and_bad_tuning(rpx, orc, rng):
+    n = 20_001
+    x = rng.integers(-9, 9, n + 1).astype(np.float64)
+    dx = dev(rpx, x)
+    y = rpx.DeviceBuffer(8 * (n + 1))
+    s, sb = scan_scratch(rpx, n)
+    # one-double offsets: the 16-byte row loads are not legal, rows go scalar
+    rpx.lib().rpx_scan_exclusive_f64(dx.ptr + 8, y.ptr + 8, n, s.ptr, sb, None, None)
+    assert np.array_equal(y.to_array(count=n, offset_bytes=8), orc.scan(x[1:]))
+    with pytest.raises(rpx.RpxError):
+        rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(0, 24), None)
     assert rpx.lib().rpx_scan_scratch_status(s.ptr) == 0
 
 
@@ -83,7 +98,8 @@ def test_jacobi_bit_exact(rpx, orc, rng, N, tsteps):
 
 @pytest.mark.parametrize("N", [3, 4, 6, 8, 10, 130, 250, 251, 1026, 2050])
 @pytest.mark.parametrize("tsteps", [1, 2, 3, 5, 6])
-@pytest.mark.parametrize("rows,depth", [(8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2), (8, 3), (16, 3)])
+@pytest.mark.parametrize("rows,depth", [(4, 1), (8, 1), (16, 1), (32, 1), (4, 2), (8, 2), (12, 2), (16, 2), (32, 2),
+                                        (4, 3), (8, 3), (12, 3), (16, 3)])
 def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows, depth):
     A0 = rng.random(N * N)
     B0 = rng.random(N * N)
@@ -108,3 +124,25 @@ def test_jacobi_tunings(rpx, orc, rng, rows):
     An = A0.reshape(N, N)
     Bref[1:-1, 1:-1] = 0.2 * ((((An[1:-1, 1:-1] + An[1:-1, :-2]) + An[1:-1, 2:]) + An[2:, 1:-1]) + An[:-2, 1:-1])
     assert np.array_equal(dB.to_array(), Bref.reshape(-1))
+
+
+@pytest.mark.parametrize("N", [6, 130, 1026])
+@pytest.mark.parametrize("depth", [1, 2, 3])
+@pytest.mark.parametrize("rows", [4, 8, 16])
+def test_jacobi_launch_sequence_skip_b(rpx, orc, rng, N, depth, rows):
+    """A run of fused launches where only the last stores B (RPX_TUNE_JACOBI_SKIP_B
+    on the others) ends in the same A and B as the two-sweep oracle."""
+    launches = 3
+    A0, B0 = rng.random(N * N), rng.random(N * N)
+    grids = [dev(rpx, A0), rpx.DeviceBuffer(8 * N * N)]
+    dB = dev(rpx, B0)
+    L = rpx.lib()
+    L.rpx_jacobi2d_copy_boundary_f64(grids[1].ptr, grids[0].ptr, N, None)
+    for k in range(launches):
+        flags = rpx.RPX_TUNE_JACOBI_SKIP_B if k < launches - 1 else 0
+        L.rpx_jacobi2d_fused_launch_f64(grids[1].ptr, dB.ptr, grids[0].ptr, N, depth,
+                                        rpx.tuning(0, rows, 0, flags), None)
+        grids.reverse()
+    A, B = orc.jacobi2d(A0, B0, N, launches * depth)
+    assert np.array_equal(grids[0].to_array(), A)
+    assert np.array_equal(dB.to_array(), B)
