@@ -69,9 +69,14 @@ void MASS3DPA::tearDown(VariantID, size_t) {
   deallocData(m_X);
   deallocData(m_Y);
 }
-void MASS3DPA::runMI355XVariant(VariantID, size_t) {
+void MASS3DPA::setMI355XTuningDefinitions(VariantID vid) {
+  addVariantTuningName(vid, "default");
+  addVariantTuningName(vid, "naive");  // one element per Q1D x Q1D workgroup, the plain FP64 comparator
+}
+void MASS3DPA::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
-  const rpx_tuning tune = defaultTuning();
+  rpx_tuning tune = defaultTuning();
+  if (tune_idx == 1) tune.flags |= RPX_TUNE_NAIVE;
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep)
     rpxCheck(rpx_mass3dpa_f64(m_B, m_Bt, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, &tune, nullptr), "MASS3DPA");

@@ -12,6 +12,7 @@ namespace apps {
 class MASS3DPA : public KernelBase {
  public:
   explicit MASS3DPA(const RunParams& params);
+  void setMI355XTuningDefinitions(VariantID vid) override;
   void setUp(VariantID vid, size_t tune_idx) override;
   void updateChecksum(VariantID vid, size_t tune_idx) override;
   void tearDown(VariantID vid, size_t tune_idx) override;

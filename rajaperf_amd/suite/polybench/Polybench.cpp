@@ -79,7 +79,8 @@ POLYBENCH_JACOBI_2D::POLYBENCH_JACOBI_2D(const RunParams& params) : KernelBase(P
   setVariantDefined(MI355X_Native);
 }
 void POLYBENCH_JACOBI_2D::setMI355XTuningDefinitions(VariantID vid) {
-  addVariantTuningName(vid, "default");    // two time steps per fused launch, 16-row chunks
+  addVariantTuningName(vid, "default");    // three time steps per fused launch, 8-row tiles
+  addVariantTuningName(vid, "depth_2");    // two time steps per fused launch
   addVariantTuningName(vid, "depth_1");    // one time step per fused launch
   addVariantTuningName(vid, "two_sweep");  // one launch per sweep, as the suite's kernels
   addVariantTuningName(vid, "two_sweep_rows_32");
@@ -111,9 +112,10 @@ void POLYBENCH_JACOBI_2D::tearDown(VariantID, size_t) {
 void POLYBENCH_JACOBI_2D::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
   rpx_tuning tune = defaultTuning();
-  const bool fused = tune_idx <= 1;
-  if (tune_idx == 1) tune.depth = 1;
-  if (tune_idx == 3) tune.items_per_thread = 32;
+  const bool fused = tune_idx <= 2;
+  if (tune_idx == 1) tune.depth = 2;
+  if (tune_idx == 2) tune.depth = 1;
+  if (tune_idx == 4) tune.items_per_thread = 32;
   // POLYBENCH_JACOBI_2D_DATA_SETUP copies the initial grids before the timer.
   const Size_type bytes = Size_type(m_N * m_N) * sizeof(Real_type);
   rpxCheck(rpx_memcpy_d2d(m_A, m_Ainit, bytes, nullptr), "JACOBI_2D copy");
