@@ -179,7 +179,11 @@ int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n,
                            void* scratch, size_t scratch_bytes,
                            const rpx_tuning* tune, void* stream);
 /* Segmented form: segment s covers [offsets[s], offsets[s+1]) and is scanned
- * independently; `offsets_host` is a host array of nseg+1 ascending values. */
+ * independently; `offsets_host` is a host array of nseg+1 ascending values.
+ * All segments run in one launch (tiles of 16384 doubles, the look-back of
+ * a tile stops at its segment's first tile).  The call uploads the segment
+ * tables into `scratch` and waits for that copy (a host synchronisation of
+ * `stream`); the scratch needs no initialisation. */
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host,
                                         int64_t nseg);
 int rpx_scan_exclusive_segmented_f64(const double* x, double* y,
@@ -264,10 +268,30 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B,
  * (src/apps/MASS3DPA.hpp:158-339, MASS3DPA-Seq.cpp:29-94).
  * B is read as B[q + q1d*d]; Bt is read LITERALLY as Bt[q + d1d*d], the
  * suite's indexing (MASS3DPA.hpp:175).  X, Y: [e][dz][dy][dx];
- * D: [e][qz][qy][qx].  Supported: 1 <= d1d <= 8, d1d <= q1d <= 9. */
+ * D: [e][qz][qy][qx].  Supported: 1 <= d1d <= 8, q1d in {d1d, d1d+1,
+ * d1d+2}, q1d <= 9.  Bit-identical to Base_Seq.  RPX_TUNE_NAIVE selects the
+ * comparator in the suite's Base_HIP shape (MASS3DPA-Hip.cpp:25-117): one
+ * element per q1d x q1d workgroup. */
 int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D,
                      const double* X, double* Y, int64_t NE, int d1d,
                      int q1d, const rpx_tuning* tune, void* stream);
+
+/* The same operator with the basis contractions on int8 matrix cores: the
+ * forward and backward Kronecker products of the bases (q1d^3 x d1d^3 and
+ * d1d^3 x q1d^3) applied as error-free int8 slice products (`nslices`
+ * digits of 7 bits, 0 = 7) with int32 accumulation, FP64 recombination, FP64
+ * quadrature scaling and FP64 Y +=.  Each output is within about
+ * (nslices+3) 2^(-7 nslices) of the FP64 result relative to
+ * |Bt'|^3 (|D| .* (|B|^3 |X|)).  No counterpart in the suite (the mixed
+ * precision path of MASS3DPA-Hip.cpp's contractions).  Supported:
+ * 1 <= d1d <= 4, d1d <= q1d <= min(5, d1d+2).  `scratch` (query
+ * rpx_mass3dpa_mixed_scratch_bytes) receives the sliced Kronecker matrices;
+ * it needs no initialisation. */
+size_t rpx_mass3dpa_mixed_scratch_bytes(int d1d, int q1d, int nslices);
+int rpx_mass3dpa_mixed_f64(const double* B, const double* Bt, const double* D,
+                           const double* X, double* Y, int64_t NE, int d1d,
+                           int q1d, int nslices, void* scratch,
+                           size_t scratch_bytes, void* stream);
 
 /* Diffusion operator action with the suite's half-storage basis tables
  * (src/apps/DIFFUSION3DPA.hpp:216-458, DIFFUSION3DPA-Seq.cpp:28-110).

@@ -29,6 +29,8 @@
 //    in the scratch header and the kernel still runs to completion.
 #include "rpx_common.hpp"
 
+#include <vector>
+
 using namespace rpx;
 
 namespace {
@@ -73,12 +75,25 @@ __device__ __forceinline__ void store_value(double* p, double v) {
                      __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int ROWS, bool NT>  // 16-byte rows per lane: a wavefront scans 128*ROWS doubles
+// Segment tables of a segmented scan: segment s covers [off[s], off[s+1])
+// and owns the global tiles [tile0[s], tile0[s+1]); tile t belongs to
+// segment tile_seg[t].
+struct SegTables {
+  const int64_t* off;
+  const int64_t* tile0;
+  const int* tile_seg;
+};
+
+// SEG = false: one array of n elements, tile t covers [t*T, t*T + T), flags
+// carry the epoch read from the header.  SEG = true: tiles come from the
+// segment tables, the look-back stops at the segment's first tile, and the
+// flags were cleared before the launch (epoch 0, no epoch advance).
+template <int ROWS, bool NT, bool SEG>  // 16-byte rows per lane: a wavefront scans 128*ROWS doubles
 __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, int64_t n, int64_t ntiles,
                                                    ScanHeader* __restrict__ hdr,
                                                    unsigned long long* __restrict__ flags,
                                                    double* __restrict__ aggs, double* __restrict__ incls,
-                                                   int vec) {
+                                                   int vec, SegTables seg) {
   constexpr int kRowLen = 2 * kWave;            // doubles per wavefront row
   constexpr int kChunk = kRowLen * ROWS;        // doubles per wavefront
   constexpr int kTileN = kChunk * (kBS / kWave);
@@ -89,11 +104,26 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
-  if (tid == 0) s_epoch = __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!SEG && tid == 0) s_epoch = __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
   const int64_t t = blockIdx.x;
-  const int64_t base = t * kTileN + int64_t(wave) * kChunk + 2 * lane;
-  const bool full = vec && (t + 1) * kTileN <= n;
+  int64_t tbeg, tend, first;  // the tile's elements [tbeg, tend); its segment's first tile
+  if (SEG) {
+    const int sg = seg.tile_seg[t];
+    first = seg.tile0[sg];
+    tbeg = seg.off[sg] + (t - first) * kTileN;
+    const int64_t send = seg.off[sg + 1];
+    tend = (tbeg + kTileN < send) ? tbeg + kTileN : send;
+  } else {
+    first = 0;
+    tbeg = t * kTileN;
+    tend = (tbeg + kTileN < n) ? tbeg + kTileN : n;
+  }
+  const int64_t base = tbeg + int64_t(wave) * kChunk + 2 * lane;
+  // x and y are 16-byte aligned (vec), so rows are too when the tile starts
+  // on an even element.
+  const bool full = vec && tend - tbeg == kTileN && (tbeg & 1) == 0;
+  n = tend;  // bounds of the partial-tile paths below
 
   // Each wavefront owns a contiguous chunk, read as ROWS coalesced 1 KiB rows.
   v2d v[ROWS];
@@ -128,7 +158,7 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
   }
   if (lane == 0) wave_tot[wave] = carry;
   __syncthreads();
-  const unsigned long long E = s_epoch;
+  const unsigned long long E = SEG ? 0ull : s_epoch;
   double wave_off = 0.0, tile_agg = 0.0;
 #pragma unroll
   for (int w = 0; w < kBS / kWave; ++w) {
@@ -139,10 +169,10 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
 
   // Publish the aggregate, then look back (wavefront 0).
   if (wave == 0) {
-    if (t == 0) {
+    if (t == first) {
       if (lane == 0) {
-        store_value(&incls[0], tile_agg);
-        store_flag(&flags[0], (E << 2) | 2ull);
+        store_value(&incls[t], tile_agg);
+        store_flag(&flags[t], (E << 2) | 2ull);
         s_prefix = 0.0;
       }
     } else {
@@ -155,8 +185,8 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
       bool expired = false;
       while (true) {
         const int64_t k = top - lane;
-        unsigned status = 2;  // before tile 0: an inclusive prefix of zero
-        if (k >= 0) {
+        unsigned status = 2;  // before the first tile: an inclusive prefix of zero
+        if (k >= first) {
           unsigned spins = 0;
           while (true) {
             const unsigned long long f = load_flag(&flags[k]);
@@ -174,7 +204,7 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
         const unsigned long long inc_mask = __ballot(status == 2);
         const int first_inc = inc_mask ? (__ffsll(static_cast<long long>(inc_mask)) - 1) : kWave;
         double val = 0.0;
-        if (k >= 0 && !expired) {
+        if (k >= first && !expired) {
           if (lane < first_inc) val = load_value(&aggs[k]);
           else if (lane == first_inc) val = load_value(&incls[k]);
         }
@@ -190,7 +220,7 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
       }
     }
     // Every tile has read the epoch once the last tile's look-back is done.
-    if (lane == 0 && t == ntiles - 1) atomicAdd(&hdr->epoch, 1ull);
+    if (!SEG && lane == 0 && t == ntiles - 1) atomicAdd(&hdr->epoch, 1ull);
   }
   __syncthreads();
 
@@ -232,13 +262,14 @@ Layout layout_for(int64_t ntiles) {
   return l;
 }
 
-template <int ROWS>
+template <int ROWS, bool SEG>
 void launch_rows(bool nt, unsigned grid, const double* x, double* y, int64_t n, int64_t ntiles, ScanHeader* hdr,
-                 unsigned long long* flags, double* aggs, double* incls, int vec, hipStream_t s) {
+                 unsigned long long* flags, double* aggs, double* incls, int vec, SegTables seg, hipStream_t s) {
   if (nt)
-    scan_kernel<ROWS, true><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec);
+    scan_kernel<ROWS, true, SEG><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec, seg);
   else
-    scan_kernel<ROWS, false><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec);
+    scan_kernel<ROWS, false, SEG><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec,
+                                                                   seg);
 }
 
 int scan_one(const double* x, double* y, int64_t n, int ipt, bool nt, void* scratch, size_t scratch_bytes,
@@ -256,13 +287,38 @@ int scan_one(const double* x, double* y, int64_t n, int ipt, bool nt, void* scra
   auto* incls = reinterpret_cast<double*>(base + l.incls_off);
   const int vec = aligned16(x) && aligned16(y);
   const unsigned grid = unsigned(ntiles);
-  if (ipt == 16) launch_rows<8>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, s);
-  else if (ipt == 32) launch_rows<16>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, s);
-  else launch_rows<32>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, s);
+  const SegTables none{nullptr, nullptr, nullptr};
+  if (ipt == 16) launch_rows<8, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+  else if (ipt == 32) launch_rows<16, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+  else launch_rows<32, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
   return launch_status();
 }
 
+// Segmented scan: every tile of every segment in one launch (tiles of
+// 256 x 64 doubles; a segment's last tile may be partial).  Scratch: the
+// single-array layout for the total tile count, then the segment tables.
+constexpr int64_t kSegTile = int64_t(kBS) * kDefaultIPT;
+
+struct SegLayout {
+  Layout l;
+  int64_t ntiles;
+  size_t tile_seg_off, off_off, tile0_off, total;
+};
+SegLayout seg_layout(const int64_t* offsets_host, int64_t nseg) {
+  SegLayout g;
+  g.ntiles = 0;
+  for (int64_t s = 0; s < nseg; ++s) g.ntiles += ceil_div(offsets_host[s + 1] - offsets_host[s], kSegTile);
+  g.l = layout_for(g.ntiles > 0 ? g.ntiles : 1);
+  g.tile_seg_off = g.l.total;
+  g.off_off = g.tile_seg_off + align256(size_t(g.ntiles) * sizeof(int));
+  g.tile0_off = g.off_off + align256(size_t(nseg + 1) * sizeof(int64_t));
+  g.total = g.tile0_off + align256(size_t(nseg + 1) * sizeof(int64_t));
+  return g;
+}
+
 }  // namespace
+
+extern "C" {}  // namespace
 
 extern "C" {
 
@@ -280,29 +336,55 @@ int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n, void* scratch,
 }
 
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host, int64_t nseg) {
-  if (!offsets_host || nseg < 0) return 0;
-  int64_t maxlen = 1;
-  for (int64_t s = 0; s < nseg; ++s) {
-    const int64_t len = offsets_host[s + 1] - offsets_host[s];
-    if (len > maxlen) maxlen = len;
-  }
-  return rpx_scan_scratch_bytes(maxlen, nullptr);
+  if (!offsets_host || nseg < 0 || nseg > INT32_MAX) return 0;
+  for (int64_t s = 0; s < nseg; ++s)
+    if (offsets_host[s + 1] < offsets_host[s]) return 0;
+  return seg_layout(offsets_host, nseg).total;
 }
 
 int rpx_scan_exclusive_segmented_f64(const double* x, double* y, const int64_t* offsets_host, int64_t nseg,
                                      void* scratch, size_t scratch_bytes, void* stream) {
-  if (!offsets_host || nseg < 0 || !scratch) return RPX_ERR_INVALID_ARGUMENT;
+  if (!offsets_host || nseg < 0 || nseg > INT32_MAX || !scratch) return RPX_ERR_INVALID_ARGUMENT;
   for (int64_t s = 0; s < nseg; ++s)
     if (offsets_host[s + 1] < offsets_host[s] || offsets_host[s] < 0) return RPX_ERR_INVALID_ARGUMENT;
   if (nseg > 0 && offsets_host[nseg] > 0 && (!x || !y)) return RPX_ERR_INVALID_ARGUMENT;
-  // Segments run back to back on the stream; each reuses the same scratch,
-  // which is legal because consecutive launches on one stream are ordered.
+  const SegLayout g = seg_layout(offsets_host, nseg);
+  if (scratch_bytes < g.total) return RPX_ERR_SCRATCH_TOO_SMALL;
+  if (g.ntiles == 0) return RPX_SUCCESS;
+  if (g.ntiles > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  hipStream_t st = as_stream(stream);
+  char* base = reinterpret_cast<char*>(scratch);
+  // Tables: each tile's segment and each segment's first tile.
+  std::vector<int> tile_seg(size_t(g.ntiles));
+  std::vector<int64_t> tile0(size_t(nseg + 1));
+  int64_t t = 0;
   for (int64_t s = 0; s < nseg; ++s) {
-    const int64_t b = offsets_host[s], len = offsets_host[s + 1] - b;
-    const int rc = scan_one(x + b, y + b, len, kDefaultIPT, true, scratch, scratch_bytes, as_stream(stream));
-    if (rc != RPX_SUCCESS) return rc;
+    tile0[s] = t;
+    const int64_t k = ceil_div(offsets_host[s + 1] - offsets_host[s], kSegTile);
+    for (int64_t i = 0; i < k; ++i) tile_seg[size_t(t + i)] = int(s);
+    t += k;
   }
-  return RPX_SUCCESS;
+  tile0[nseg] = t;
+  auto* d_tile_seg = reinterpret_cast<int*>(base + g.tile_seg_off);
+  auto* d_off = reinterpret_cast<int64_t*>(base + g.off_off);
+  auto* d_tile0 = reinterpret_cast<int64_t*>(base + g.tile0_off);
+  RPX_TRY(hipMemcpyAsync(d_tile_seg, tile_seg.data(), tile_seg.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  RPX_TRY(hipMemcpyAsync(d_off, offsets_host, size_t(nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  RPX_TRY(hipMemcpyAsync(d_tile0, tile0.data(), tile0.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  // The host tables must outlive the copies.
+  RPX_TRY(hipStreamSynchronize(st));
+  // Flags start at zero for every call (no epochs across segments).
+  auto* flags = reinterpret_cast<unsigned long long*>(base + g.l.flags_off);
+  const int64_t words = g.ntiles * 2;
+  zero_kernel<<<dim3(unsigned(ceil_div(words, 256) < 4096 ? ceil_div(words, 256) : 4096)), dim3(256), 0, st>>>(
+      reinterpret_cast<unsigned*>(flags), words);
+  if (const int rc = launch_status(); rc != RPX_SUCCESS) return rc;
+  const SegTables seg{d_off, d_tile0, d_tile_seg};
+  const int vec = aligned16(x) && aligned16(y);
+  launch_rows<32, true>(true, unsigned(g.ntiles), x, y, offsets_host[nseg], g.ntiles,
+                        reinterpret_cast<ScanHeader*>(base), flags, reinterpret_cast<double*>(base + g.l.aggs_off),
+                        reinterpret_cast<double*>(base + g.l.incls_off), vec, seg, st);
+  return launch_status();
 }
 
 // Zeroes a scan scratch buffer; required once before its first use.

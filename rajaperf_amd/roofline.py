@@ -14,7 +14,8 @@ arithmetic on the host.
 Configurations (BASELINE workloads):
   2  TRIAD/DOT/REDUCE_SUM/SCAN (+COPY/MUL/ADD) at N=2^28; JACOBI_2D 8192^2
   3  MASS3DPA and DIFFUSION3DPA, NE=262,144, D1D=4, Q1D=5
-  4  GEMM 4096^3 (FP64 MFMA, the plain FP64 comparator, mixed precision)
+  4  GEMM 4096^3 (FP64 MFMA, the plain FP64 comparator, mixed precision);
+     MASS3DPA of config 3 with its contractions on int8 MFMA
   5  SORT and SCAN over 256 segments, lengths log-uniform in [2^10, 2^20]
 """
 
@@ -198,6 +199,51 @@ def config4(rows, reps, sample_rows=8):
     _row(rows, "Polybench_GEMM(mixed-int8x7)", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
          err_vs_exact=err_mix, int8_tops=round(28 * flops / (ms_mix / 1e3) / 1e12, 1),
          speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
+    del dA, dB, Cf, Cm, scr
+    mass_mixed(rows, reps)
+
+
+def mass_mixed(rows, reps):
+    """MASS3DPA of config 3 with the basis contractions on int8 MFMA, against
+    the FP64 kernel and the plain FP64 comparator.  Error per output relative
+    to the operator applied to |B|, |Bt|, D, |X| (its magnitude bound), with
+    the FP64 kernel's result as reference."""
+    L = abi.lib()
+    NE, D, Q = 262144, 4, 5
+    B = abi.DeviceBuffer
+    Bm, Bt, Ba, Bta = B(8 * Q * D), B(8 * Q * D), B(8 * Q * D), B(8 * Q * D)
+    L.rpx_fill_uniform_f64(Bm.ptr, Q * D, SEED, 0, -1.0, 1.0, None)
+    L.rpx_fill_uniform_f64(Bt.ptr, Q * D, SEED, 1, -1.0, 1.0, None)
+    Ba.upload(np.abs(Bm.to_array()))
+    Bta.upload(np.abs(Bt.to_array()))
+    Dm = B(8 * NE * Q ** 3)
+    X, Xa = B(8 * NE * D ** 3), B(8 * NE * D ** 3)
+    L.rpx_fill_uniform_f64(Dm.ptr, NE * Q ** 3, SEED, 2, 0.5, 2.0, None)
+    L.rpx_fill_uniform_f64(X.ptr, NE * D ** 3, SEED, 3, -1.0, 1.0, None)
+    Xa.upload(np.abs(X.to_array()))
+    Yf, Ym, Yz = B(8 * NE * D ** 3), B(8 * NE * D ** 3), B(8 * NE * D ** 3)
+    # The suite's bytes and FLOPs per rep (MASS3DPA.cpp:38-49), as in config 3.
+    nbytes = 2 * 8 * Q * D + 2 * 8 * D ** 3 * NE + 8 * Q ** 3 * NE + 8 * D ** 3 * NE
+    flops = NE * (2 * D ** 3 * Q + 2 * D * D * Q * Q + 2 * D * Q ** 3 + Q ** 3 + 2 * Q ** 3 * D + 2 * Q * Q * D * D
+                  + 2 * Q * D ** 3 + D ** 3)
+    sb = L.rpx_mass3dpa_mixed_scratch_bytes(D, Q, 7)
+    scr = B(sb)
+    naive = abi.tuning(flags=abi.RPX_TUNE_NAIVE)
+    ms_f64 = _time(lambda: L.rpx_mass3dpa_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Yz.ptr, NE, D, Q, None, None), reps)
+    ms_naive = _time(lambda: L.rpx_mass3dpa_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Yz.ptr, NE, D, Q, naive, None), reps)
+    ms_mix = _time(lambda: L.rpx_mass3dpa_mixed_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Yz.ptr, NE, D, Q, 7, scr.ptr, sb,
+                                                    None), reps)
+    # One application each from Y = 0: FP64 reference, mixed, and the bound.
+    for y in (Yf, Ym, Yz):
+        L.rpx_fill_const_f64(y.ptr, NE * D ** 3, 0.0, None)
+    L.rpx_mass3dpa_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Yf.ptr, NE, D, Q, None, None)
+    L.rpx_mass3dpa_mixed_f64(Bm.ptr, Bt.ptr, Dm.ptr, X.ptr, Ym.ptr, NE, D, Q, 7, scr.ptr, sb, None)
+    L.rpx_mass3dpa_f64(Ba.ptr, Bta.ptr, Dm.ptr, Xa.ptr, Yz.ptr, NE, D, Q, None, None)
+    yf, ym, mag = Yf.to_array(), Ym.to_array(), Yz.to_array()
+    err = float(np.max(np.abs(ym - yf) / mag))
+    _row(rows, "Apps_MASS3DPA(mixed-int8x7)", 4, ms_mix, nbytes, flops, err_vs_fp64=err,
+         speedup_vs_fp64=round(ms_f64 / ms_mix, 3), speedup_vs_plain_fp64=round(ms_naive / ms_mix, 3),
+         fp64_ms=round(ms_f64, 5), plain_fp64_ms=round(ms_naive, 5))
 
 
 def config5(rows, reps):

@@ -1,7 +1,9 @@
-"""GPU: mixed-precision GEMM (exact int8 slice products on MFMA, FP64
-recombination).  Error is measured per element against an 80-bit
-(long double) reference, relative to |alpha| (|A||B|)_ij, the conditioning
-of each entry; the stated tolerance is 1e-12 at the default 7 slices."""
+"""GPU: mixed-precision GEMM and MASS3DPA (exact int8 slice products on
+MFMA, FP64 recombination).  GEMM error is measured per element against an
+80-bit (long double) reference, relative to |alpha| (|A||B|)_ij, the
+conditioning of each entry; MASS3DPA error against the FP64 oracle relative
+to the operator applied to absolute values.  The stated tolerance is 1e-12
+at the default 7 slices."""
 
 import numpy as np
 import pytest
@@ -70,3 +72,73 @@ def test_mixed_gemm_exact_on_small_integers(rpx, rng):
     B = rng.integers(-8, 8, nk * nj).astype(np.float64)
     C = run_mixed(rpx, A, B, 1.0, ni, nj, nk)
     assert np.array_equal(C, A.reshape(ni, nk) @ B.reshape(nk, nj))
+
+
+# ---- MASS3DPA on int8 matrix cores -------------------------------------------
+
+MIXED_MASS_ORDERS = [(d, q) for d in range(1, 5) for q in (d, d + 1, d + 2) if q <= 5]
+
+
+def _mass_mixed(rpx, bufs, NE, d1d, q1d, nslices=7):
+    L = rpx.lib()
+    sb = L.rpx_mass3dpa_mixed_scratch_bytes(d1d, q1d, nslices)
+    scr = rpx.DeviceBuffer(sb)
+    L.rpx_mass3dpa_mixed_f64(*[b.ptr for b in bufs], NE, d1d, q1d, nslices, scr.ptr, sb, None)
+
+
+@pytest.mark.parametrize("d1d,q1d", MIXED_MASS_ORDERS)
+def test_mass3dpa_mixed_meets_1e12(rpx, orc, rng, d1d, q1d):
+    """Error relative to |Bt'|^3 (|D| . |B|^3 |X|), the magnitude bound of
+    every output, against the FP64 Base_Seq oracle; NE = 67 leaves a partial
+    workgroup of elements."""
+    NE = 67
+    B, Bt = rng.uniform(-1, 1, q1d * d1d), rng.uniform(-1, 1, q1d * d1d)
+    D = rng.uniform(0.5, 2, NE * q1d ** 3)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    bufs = [dev(rpx, a) for a in (B, Bt, D, X, np.zeros(NE * d1d ** 3))]
+    _mass_mixed(rpx, bufs, NE, d1d, q1d)
+    zeros = np.zeros(NE * d1d ** 3)
+    ref = orc.mass3dpa(B, Bt, D, X, zeros, NE, d1d, q1d)
+    mag = orc.mass3dpa(np.abs(B), np.abs(Bt), D, np.abs(X), zeros, NE, d1d, q1d)
+    err = np.max(np.abs(bufs[4].to_array() - ref) / mag)
+    assert err <= 1e-12, err
+
+
+def test_mass3dpa_mixed_accumulates_and_slices_trade_accuracy(rpx, orc, rng):
+    NE, d1d, q1d = 100, 4, 5
+    B, Bt = rng.uniform(-1, 1, 20), rng.uniform(-1, 1, 20)
+    D = rng.uniform(0.5, 2, NE * 125)
+    X = rng.uniform(-1, 1, NE * 64)
+    Y0 = rng.uniform(-1, 1, NE * 64)
+    zeros = np.zeros(NE * 64)
+    mag = orc.mass3dpa(np.abs(B), np.abs(Bt), D, np.abs(X), zeros, NE, d1d, q1d)
+    ref = orc.mass3dpa(B, Bt, D, X, Y0, NE, d1d, q1d)
+    errs = {}
+    for s in (3, 7):
+        bufs = [dev(rpx, a) for a in (B, Bt, D, X, Y0)]
+        _mass_mixed(rpx, bufs, NE, d1d, q1d, s)
+        # Y += keeps Y0; the final FP64 add may round once more than the oracle's
+        errs[s] = np.max(np.abs(bufs[4].to_array() - ref) / (mag + np.spacing(np.abs(ref))))
+    assert errs[7] <= 1e-12
+    assert 1e-9 < errs[3] <= 1e-5  # three 7-bit digits: about 6 * 2^-21
+
+
+def test_mass3dpa_mixed_exact_on_all_ones(rpx):
+    # The suite's reference inputs (MASS3DPA.cpp:76-84): every product is exact.
+    NE, d1d, q1d = 1000, 4, 5
+    bufs = [dev(rpx, a) for a in (np.ones(20), np.ones(20), np.ones(NE * 125), np.ones(NE * 64),
+                                  np.zeros(NE * 64))]
+    _mass_mixed(rpx, bufs, NE, d1d, q1d)
+    assert np.all(bufs[4].to_array() == 8000.0)
+
+
+def test_mass3dpa_mixed_rejects_orders_and_small_scratch(rpx):
+    L = rpx.lib()
+    assert L.rpx_mass3dpa_mixed_scratch_bytes(5, 6, 7) == 0
+    bufs = [rpx.DeviceBuffer(8 * 1000) for _ in range(5)]
+    sb = L.rpx_mass3dpa_mixed_scratch_bytes(4, 5, 7)
+    scr = rpx.DeviceBuffer(sb)
+    with pytest.raises(rpx.RpxError):
+        L.rpx_mass3dpa_mixed_f64(*[b.ptr for b in bufs], 1, 5, 6, 7, scr.ptr, sb, None)
+    with pytest.raises(rpx.RpxError):
+        L.rpx_mass3dpa_mixed_f64(*[b.ptr for b in bufs], 1, 4, 5, 7, scr.ptr, sb - 256, None)

@@ -77,11 +77,35 @@ def test_scan_in_place_and_segmented(rpx, orc, rng):
     o = rpx.offsets_array(offs)
     sb2 = rpx.lib().rpx_scan_segmented_scratch_bytes(o, len(offs) - 1)
     s2 = rpx.DeviceBuffer(sb2)
-    rpx.lib().rpx_scan_scratch_init(s2.ptr, sb2, None)
     rpx.lib().rpx_scan_exclusive_segmented_f64(d2.ptr, y.ptr, o, len(offs) - 1, s2.ptr, sb2, None)
     got = y.to_array()
     for b, e in zip(offs[:-1], offs[1:]):
         assert np.array_equal(got[b:e], orc.scan(x[b:e]))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_scan_segmented_ragged_one_launch(rpx, orc, seed):
+    """Ragged segments (empty, odd offsets, multi-tile) in one launch, twice on
+    the same scratch; exact on integer-valued inputs."""
+    rng = np.random.default_rng(seed)
+    lens = np.exp(rng.uniform(0, np.log(70_000), 97)).astype(np.int64)
+    lens[::13] = 0
+    offs = np.concatenate([[0], np.cumsum(lens)]).tolist()
+    n = offs[-1]
+    x = rng.integers(-50, 50, n).astype(np.float64)
+    dx, y = dev(rpx, x), rpx.DeviceBuffer(8 * n)
+    o = rpx.offsets_array(offs)
+    L = rpx.lib()
+    sb = L.rpx_scan_segmented_scratch_bytes(o, len(offs) - 1)
+    scr = rpx.DeviceBuffer(sb)
+    for _ in range(2):
+        L.rpx_scan_exclusive_segmented_f64(dx.ptr, y.ptr, o, len(offs) - 1, scr.ptr, sb, None)
+        got = y.to_array()
+        for b, e in zip(offs[:-1], offs[1:]):
+            assert np.array_equal(got[b:e], orc.scan(x[b:e]))
+    assert L.rpx_scan_scratch_status(scr.ptr) == 0
+    with pytest.raises(rpx.RpxError):  # scratch too small is an error, not a fault
+        L.rpx_scan_exclusive_segmented_f64(dx.ptr, y.ptr, o, len(offs) - 1, scr.ptr, sb - 256, None)
 
 
 @pytest.mark.parametrize("N", [3, 4, 5, 8, 33, 34, 130, 514, 1026, 2050])
