@@ -276,6 +276,38 @@ int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D,
                      const double* X, double* Y, int64_t NE, int d1d,
                      int q1d, const rpx_tuning* tune, void* stream);
 
+/* Ragged batches in one persistent launch (the suite runs one fixed order
+ * and size per kernel object; MASS3DPA.cpp:26-31).  Batch i is one operator
+ * application, as rpx_mass3dpa_f64 with its own bases, data, element count
+ * and order; orders may differ between batches (1 <= d1d <= 8, q1d =
+ * d1d + 1).  Results are bit-identical to calling rpx_mass3dpa_f64 per batch.
+ * rpx_mass3dpa_batched_plan uploads the batch and chunk tables into
+ * `scratch` and waits for that copy (a host synchronisation of `stream`);
+ * rpx_mass3dpa_batched_run then launches the persistent kernel on a planned
+ * scratch any number of times (no host synchronisation; it may be captured
+ * in a graph).  rpx_mass3dpa_batched_f64 does both.  tune->grid_size sets
+ * the number of persistent workgroups (default 3 per CU). */
+typedef struct rpx_mass_batch {
+  const double* B;
+  const double* Bt;
+  const double* D;
+  const double* X;
+  double* Y;
+  int64_t ne;
+  int32_t d1d;
+  int32_t q1d;
+} rpx_mass_batch;
+size_t rpx_mass3dpa_batched_scratch_bytes(const rpx_mass_batch* batches_host,
+                                          int nbatch);
+int rpx_mass3dpa_batched_plan(const rpx_mass_batch* batches_host, int nbatch,
+                              void* scratch, size_t scratch_bytes,
+                              void* stream);
+int rpx_mass3dpa_batched_run(const void* scratch, const rpx_tuning* tune,
+                             void* stream);
+int rpx_mass3dpa_batched_f64(const rpx_mass_batch* batches_host, int nbatch,
+                             void* scratch, size_t scratch_bytes,
+                             const rpx_tuning* tune, void* stream);
+
 /* The same operator with the basis contractions on int8 matrix cores: the
  * forward and backward Kronecker products of the bases (q1d^3 x d1d^3 and
  * d1d^3 x q1d^3) applied as error-free int8 slice products (`nslices`

@@ -48,6 +48,11 @@ class Tuning(C.Structure):
                 ("grid_size", C.c_int32), ("flags", C.c_int32), ("depth", C.c_int32)]
 
 
+class MassBatch(C.Structure):
+    _fields_ = [("B", C.c_void_p), ("Bt", C.c_void_p), ("D", C.c_void_p), ("X", C.c_void_p), ("Y", C.c_void_p),
+                ("ne", C.c_int64), ("d1d", C.c_int32), ("q1d", C.c_int32)]
+
+
 _vp, _i64, _u64, _d, _i32, _sz = C.c_void_p, C.c_int64, C.c_uint64, C.c_double, C.c_int, C.c_size_t
 _tp = C.POINTER(Tuning)
 _i64p = C.POINTER(C.c_int64)
@@ -104,6 +109,10 @@ _SIGS = {
     "rpx_gemm_mixed_scratch_bytes": (_sz, [_i64, _i64, _i64, _i32]),
     "rpx_gemm_mixed_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _i32, _vp, _sz, _vp]),
     "rpx_mass3dpa_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _tp, _vp]),
+    "rpx_mass3dpa_batched_scratch_bytes": (_sz, [C.POINTER(MassBatch), _i32]),
+    "rpx_mass3dpa_batched_plan": (_i32, [C.POINTER(MassBatch), _i32, _vp, _sz, _vp]),
+    "rpx_mass3dpa_batched_run": (_i32, [_vp, _tp, _vp]),
+    "rpx_mass3dpa_batched_f64": (_i32, [C.POINTER(MassBatch), _i32, _vp, _sz, _tp, _vp]),
     "rpx_mass3dpa_mixed_scratch_bytes": (_sz, [_i32, _i32, _i32]),
     "rpx_mass3dpa_mixed_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp, _sz, _vp]),
     "rpx_diffusion3dpa_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _tp, _vp]),

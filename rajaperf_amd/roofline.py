@@ -16,7 +16,8 @@ Configurations (BASELINE workloads):
   3  MASS3DPA and DIFFUSION3DPA, NE=262,144, D1D=4, Q1D=5
   4  GEMM 4096^3 (FP64 MFMA, the plain FP64 comparator, mixed precision);
      MASS3DPA of config 3 with its contractions on int8 MFMA
-  5  SORT and SCAN over 256 segments, lengths log-uniform in [2^10, 2^20]
+  5  SORT and SCAN over 256 segments, lengths log-uniform in [2^10, 2^20];
+     MASS3DPA over 64 ragged batches of orders 2..8 in one persistent launch
 """
 
 from __future__ import annotations
@@ -271,6 +272,43 @@ def config5(rows, reps):
     L.rpx_scan_scratch_init(sscr.ptr, ssb, None)
     ms = _time(lambda: L.rpx_scan_exclusive_segmented_f64(src.ptr, y.ptr, o, 256, sscr.ptr, ssb, None), reps)
     _row(rows, "Algorithm_SCAN(256 ragged)", 5, ms, 16 * n)
+    del src, x, y, scr, sscr
+    mass_batches(rows, reps)
+
+
+def mass_batches(rows, reps, nbatch=64):
+    """64 MASS3DPA batches, NE log-uniform in [1024, 65536], D1D uniform in
+    {2..8}, Q1D = D1D + 1, data as in config 3: one persistent launch against
+    one launch per batch."""
+    L = abi.lib()
+    rng = np.random.default_rng(SEED)
+    nes = np.exp(rng.uniform(np.log(1024), np.log(65536), nbatch)).astype(np.int64)
+    ds = rng.integers(2, 9, nbatch)
+    B = abi.DeviceBuffer
+    keep, arr = [], (abi.MassBatch * nbatch)()
+    nbytes = 0
+    for i, (ne, d) in enumerate(zip(nes.tolist(), ds.tolist())):
+        q = d + 1
+        bufs = [B(8 * q * d), B(8 * q * d), B(8 * ne * q ** 3), B(8 * ne * d ** 3), B(8 * ne * d ** 3)]
+        for k, (buf, n, lo, hi) in enumerate(zip(bufs, [q * d, q * d, ne * q ** 3, ne * d ** 3],
+                                                 [-1.0, -1.0, 0.5, -1.0], [1.0, 1.0, 2.0, 1.0])):
+            L.rpx_fill_uniform_f64(buf.ptr, n, SEED + i, k, lo, hi, None)
+        L.rpx_fill_const_f64(bufs[4].ptr, ne * d ** 3, 0.0, None)
+        keep.append(bufs)
+        arr[i] = abi.MassBatch(*[b.ptr for b in bufs], ne, d, q)
+        nbytes += 2 * 8 * q * d + 8 * (3 * d ** 3 + q ** 3) * ne  # MASS3DPA.cpp:38-42 per batch
+    sb = L.rpx_mass3dpa_batched_scratch_bytes(arr, nbatch)
+    scr = B(sb)
+    L.rpx_mass3dpa_batched_plan(arr, nbatch, scr.ptr, sb, None)
+    ms = _time(lambda: L.rpx_mass3dpa_batched_run(scr.ptr, None, None), reps)
+
+    def per_batch():
+        for i in range(nbatch):
+            b = arr[i]
+            L.rpx_mass3dpa_f64(b.B, b.Bt, b.D, b.X, b.Y, b.ne, b.d1d, b.q1d, None, None)
+    ms_each = _time(per_batch, reps)
+    _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one persistent launch)", 5, ms, nbytes,
+         elements=int(nes.sum()), per_batch_launches_ms=round(ms_each, 5), speedup=round(ms_each / ms, 3))
 
 
 def main(argv=None):

@@ -111,3 +111,67 @@ def test_fem_rejects_unsupported_orders(rpx):
         rpx.lib().rpx_mass3dpa_f64(*[b.ptr for b in bufs], 1, 9, 10, None, None)
     with pytest.raises(RpxError):
         rpx.lib().rpx_diffusion3dpa_f64(*[b.ptr for b in bufs], 1, 3, 5, 1, None, None)
+
+
+def _batches(rpx, rng, specs):
+    """Device data for MASS3DPA batches [(NE, d1d)], Q1D = d1d + 1."""
+    out = []
+    for ne, d in specs:
+        q = d + 1
+        host = [rng.uniform(-1, 1, q * d), rng.uniform(-1, 1, q * d), rng.uniform(0.5, 2, max(ne, 1) * q ** 3),
+                rng.uniform(-1, 1, max(ne, 1) * d ** 3), rng.uniform(-1, 1, max(ne, 1) * d ** 3)]
+        out.append((ne, d, host, [dev(rpx, a) for a in host]))
+    return out
+
+
+@pytest.mark.parametrize("grid", [0, 1, 7])
+def test_mass3dpa_ragged_batches_one_launch(rpx, orc, rng, grid):
+    """Ragged batches of mixed orders in one persistent launch equal one
+    rpx_mass3dpa_f64 call per batch, bit for bit, and the oracle."""
+    specs = [(300, 2), (0, 5), (1, 8), (2049, 4), (77, 7), (513, 3), (40, 6), (9, 1), (130, 8)]
+    data = _batches(rpx, rng, specs)
+    arr = (rpx.MassBatch * len(data))()
+    for i, (ne, d, host, bufs) in enumerate(data):
+        arr[i] = rpx.MassBatch(*[b.ptr for b in bufs], ne, d, d + 1)
+    L = rpx.lib()
+    sb = L.rpx_mass3dpa_batched_scratch_bytes(arr, len(data))
+    scr = rpx.DeviceBuffer(sb)
+    L.rpx_mass3dpa_batched_f64(arr, len(data), scr.ptr, sb, rpx.tuning(grid_size=grid), None)
+    for ne, d, host, bufs in data:
+        ref_bufs = [dev(rpx, a) for a in host]
+        if ne:
+            L.rpx_mass3dpa_f64(*[b.ptr for b in ref_bufs], ne, d, d + 1, None, None)
+        assert np.array_equal(bufs[4].to_array(), ref_bufs[4].to_array())
+        if 0 < ne <= 300:
+            ref = orc.mass3dpa(*host[:4], host[4][: ne * d ** 3], ne, d, d + 1)
+            assert np.array_equal(bufs[4].to_array()[: ne * d ** 3], ref)
+
+
+def test_mass3dpa_batches_reject_bad_orders(rpx, rng):
+    data = _batches(rpx, rng, [(10, 3)])
+    ne, d, host, bufs = data[0]
+    arr = (rpx.MassBatch * 1)(rpx.MassBatch(*[b.ptr for b in bufs], ne, d, d + 2))
+    L = rpx.lib()
+    sb = L.rpx_mass3dpa_batched_scratch_bytes(arr, 1)
+    scr = rpx.DeviceBuffer(sb)
+    with pytest.raises(rpx.RpxError):
+        L.rpx_mass3dpa_batched_f64(arr, 1, scr.ptr, sb, None, None)
+
+
+def test_mass3dpa_batched_plan_once_run_twice(rpx, rng):
+    specs = [(1000, 2), (333, 8), (64, 4)]
+    data = _batches(rpx, rng, specs)
+    arr = (rpx.MassBatch * len(data))()
+    for i, (ne, d, host, bufs) in enumerate(data):
+        arr[i] = rpx.MassBatch(*[b.ptr for b in bufs], ne, d, d + 1)
+    L = rpx.lib()
+    sb = L.rpx_mass3dpa_batched_scratch_bytes(arr, len(data))
+    scr = rpx.DeviceBuffer(sb)
+    L.rpx_mass3dpa_batched_plan(arr, len(data), scr.ptr, sb, None)
+    for _ in range(2):
+        L.rpx_mass3dpa_batched_run(scr.ptr, None, None)
+    for ne, d, host, bufs in data:
+        ref_bufs = [dev(rpx, a) for a in host]
+        for _ in range(2):
+            L.rpx_mass3dpa_f64(*[b.ptr for b in ref_bufs], ne, d, d + 1, None, None)
+        assert np.array_equal(bufs[4].to_array(), ref_bufs[4].to_array())
