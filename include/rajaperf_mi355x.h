@@ -90,6 +90,17 @@ int rpx_stream_create(void** stream_host);
 int rpx_stream_destroy(void* stream);
 int rpx_stream_synchronize(void* stream);
 int rpx_device_synchronize(void);
+/* Graph capture of a sequence of calls (the suite has none; the rep loops of
+ * Executor::runSuite, Executor.cpp:645-670, launch kernel by kernel).
+ * Between begin and end, calls on `stream` (a created stream, not the null
+ * stream) are recorded instead of run; rpx_graph_launch replays them.  Entry
+ * points that synchronise the host (the segmented SCAN and ragged SORT
+ * table uploads, rpx_mass3dpa_batched_plan, the scratch status queries)
+ * cannot be captured; every other kernel entry point can. */
+int rpx_graph_begin_capture(void* stream);
+int rpx_graph_end_capture(void* stream, void** graph_exec_host);
+int rpx_graph_launch(void* graph_exec, void* stream);
+int rpx_graph_destroy(void* graph_exec);
 int rpx_event_create(void** event_host);
 int rpx_event_destroy(void* event);
 int rpx_event_record(void* event, void* stream);

@@ -74,6 +74,10 @@ _SIGS = {
     "rpx_stream_destroy": (_i32, [_vp]),
     "rpx_stream_synchronize": (_i32, [_vp]),
     "rpx_device_synchronize": (_i32, []),
+    "rpx_graph_begin_capture": (_i32, [_vp]),
+    "rpx_graph_end_capture": (_i32, [_vp, C.POINTER(_vp)]),
+    "rpx_graph_launch": (_i32, [_vp, _vp]),
+    "rpx_graph_destroy": (_i32, [_vp]),
     "rpx_event_create": (_i32, [C.POINTER(_vp)]),
     "rpx_event_destroy": (_i32, [_vp]),
     "rpx_event_record": (_i32, [_vp, _vp]),
@@ -250,6 +254,54 @@ def synchronize() -> None:
 def offsets_array(offsets: Sequence[int]):
     arr = (C.c_int64 * len(offsets))(*[int(o) for o in offsets])
     return arr
+
+
+class Stream:
+    """A created (non-blocking) HIP stream; graphs can only capture these."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        lib().rpx_stream_create(C.byref(p))
+        self.ptr = p.value
+
+    def synchronize(self):
+        lib().rpx_stream_synchronize(self.ptr)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().rpx_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Graph:
+    """Calls made on `stream` inside `with Graph(stream) as g:` are captured;
+    g.launch() replays them on the stream."""
+
+    def __init__(self, stream: Stream):
+        self.stream = stream
+        self.ptr = None
+
+    def __enter__(self):
+        lib().rpx_graph_begin_capture(self.stream.ptr)
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        p = C.c_void_p()
+        lib().rpx_graph_end_capture(self.stream.ptr, C.byref(p))
+        self.ptr = p.value
+        return False
+
+    def launch(self):
+        lib().rpx_graph_launch(self.ptr, self.stream.ptr)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().rpx_graph_destroy(self.ptr)
+        except Exception:
+            pass
 
 
 class Event:

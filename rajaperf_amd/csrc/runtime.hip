@@ -165,6 +165,31 @@ int rpx_stream_create(void** stream_host) {
 int rpx_stream_destroy(void* stream) { return stream ? hip_status(hipStreamDestroy(as_stream(stream))) : RPX_SUCCESS; }
 int rpx_stream_synchronize(void* stream) { return hip_status(hipStreamSynchronize(as_stream(stream))); }
 int rpx_device_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
+// Stream capture into a graph.  The capture mode is thread-local, so other
+// host threads may keep issuing work while one thread captures.
+int rpx_graph_begin_capture(void* stream) {
+  if (!stream) return RPX_ERR_INVALID_ARGUMENT;  // the null stream cannot be captured
+  return hip_status(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+}
+int rpx_graph_end_capture(void* stream, void** graph_exec_host) {
+  if (!stream || !graph_exec_host) return RPX_ERR_INVALID_ARGUMENT;
+  hipGraph_t g = nullptr;
+  RPX_TRY(hipStreamEndCapture(as_stream(stream), &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);  // the executable graph keeps what it needs
+  if (e != hipSuccess) return hip_status(e);
+  *graph_exec_host = exec;
+  return RPX_SUCCESS;
+}
+int rpx_graph_launch(void* graph_exec, void* stream) {
+  if (!graph_exec) return RPX_ERR_INVALID_ARGUMENT;
+  return hip_status(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), as_stream(stream)));
+}
+int rpx_graph_destroy(void* graph_exec) {
+  return graph_exec ? hip_status(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph_exec))) : RPX_SUCCESS;
+}
+
 int rpx_event_create(void** event_host) {
   if (!event_host) return RPX_ERR_INVALID_ARGUMENT;
   hipEvent_t e = nullptr;

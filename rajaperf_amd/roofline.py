@@ -17,7 +17,8 @@ Configurations (BASELINE workloads):
   4  GEMM 4096^3 (FP64 MFMA, the plain FP64 comparator, mixed precision);
      MASS3DPA of config 3 with its contractions on int8 MFMA
   5  SORT and SCAN over 256 segments, lengths log-uniform in [2^10, 2^20];
-     MASS3DPA over 64 ragged batches of orders 2..8 in one persistent launch
+     MASS3DPA over 64 ragged batches of orders 2..8 in one persistent launch;
+     one pass of all kernels at random sizes, eager and as one HIP graph
 """
 
 from __future__ import annotations
@@ -274,6 +275,35 @@ def config5(rows, reps):
     _row(rows, "Algorithm_SCAN(256 ragged)", 5, ms, 16 * n)
     del src, x, y, scr, sscr
     mass_batches(rows, reps)
+    graph_pass(rows, reps)
+
+
+def graph_pass(rows, reps):
+    """One pass of every kernel at randomly drawn sizes, run eagerly and as
+    one captured HIP graph."""
+    from .suite_pass import SuitePass, draw_sizes
+    sizes = draw_sizes(np.random.default_rng(SEED))
+    p = SuitePass(sizes)
+    st = abi.Stream()
+    with abi.Graph(st) as graph:
+        p.run(st)
+    st.synchronize()
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        st.synchronize()
+        e0, e1 = abi.Event(), abi.Event()
+        e0.record(st.ptr)
+        for _ in range(reps):
+            fn()
+        e1.record(st.ptr)
+        st.synchronize()
+        return e0.elapsed_ms(e1) / reps
+    ms_eager = timed(lambda: p.run(st))
+    ms_graph = timed(graph.launch)
+    _row(rows, "Suite pass of all kernels (random sizes), one HIP graph", 5, ms_graph, p.bytes_per_pass(),
+         eager_ms=round(ms_eager, 5), sizes=str(sizes).replace(" ", ""))
 
 
 def mass_batches(rows, reps, nbatch=64):
