@@ -1,5 +1,7 @@
 #include "Executor.hpp"
 
+#include "Autotune.hpp"
+
 #include <sys/stat.h>
 
 #include <cmath>
@@ -52,6 +54,8 @@ void Executor::setupSuite() {  // Executor.cpp:246-357
   if (in == RunParams::InfoRequest || in == RunParams::BadInput) return;
   for (KernelID kid : run_params.getKernelIDsToRun()) kernels.push_back(getKernelObject(kid, run_params));
   for (VariantID vid : run_params.getVariantIDsToRun()) variant_ids.push_back(vid);
+  Autotuner::instance().reset();
+  if (!run_params.getTuningCache().empty()) Autotuner::instance().setCacheFile(run_params.getTuningCache());
 }
 
 bool Executor::tuningSelected(const std::string& name) const {
@@ -204,6 +208,13 @@ void Executor::writeJson(const std::string& path) const {
           << ", \"flops_per_rep\": " << k->getFLOPsPerRep() << "}";
         first = false;
       }
+  f << "], \"autotune\": [";
+  const auto& ds = Autotuner::instance().decisions();
+  for (size_t i = 0; i < ds.size(); ++i)
+    f << (i ? ", " : "") << std::setprecision(6) << "{\"kernel\": \"" << json_escape(ds[i].kernel)
+      << "\", \"shape\": \"" << json_escape(ds[i].shape) << "\", \"arch\": \"" << json_escape(ds[i].arch)
+      << "\", \"choice\": \"" << json_escape(ds[i].choice) << "\", \"trial_ms\": " << ds[i].ms_per_rep
+      << ", \"from_cache\": " << (ds[i].from_cache ? "true" : "false") << "}";
   f << "]}\n";
 }
 

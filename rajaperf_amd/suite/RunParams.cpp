@@ -94,6 +94,8 @@ void RunParams::parse(int argc, const char** argv) {
       if (need(i + 1 < argc && !is_option(argv[i + 1]))) outfile_prefix = argv[++i];
     } else if (opt == "--json") {
       if (need(i + 1 < argc && !is_option(argv[i + 1]))) json_file = argv[++i];
+    } else if (opt == "--tuning-cache") {
+      if (need(i + 1 < argc && !is_option(argv[i + 1]))) tuning_cache = argv[++i];
     } else if (opt == "--refvar" || opt == "-rv") {
       if (need(i + 1 < argc && !is_option(argv[i + 1]))) reference_variant = argv[++i];
     } else if (opt == "-k" || opt == "--kernels") {
@@ -227,6 +229,7 @@ void RunParams::printHelpMessage(std::ostream& str) const {
       << "  --input-mode reference|seeded, --seed <s>\n"
       << "  --tsteps <n> (JACOBI_2D), --nk <n> (GEMM), --d1d/--q1d <n> (FEM)\n"
       << "  -od/--outdir <dir>, -of/--outfile <prefix>, --json <file>\n"
+      << "  --tuning-cache <file>  (decisions of the \"auto\" tunings)\n"
       << "  --refvar <variant>, --dryrun, --disable-warmup, --show-progress\n"
       << "  --print-kernels, --print-variants, -h/--help\n";
 }

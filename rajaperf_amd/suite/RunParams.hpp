@@ -8,7 +8,8 @@
 // --disable-warmup, --show-progress, --print-kernels, --print-variants,
 // -h/--help.  Added for this variant: --input-mode {reference,seeded},
 // --seed, --tsteps (JACOBI_2D), --nk (GEMM), --d1d/--q1d (FEM orders),
-// --json <file> (machine-readable results).
+// --json <file> (machine-readable results), --tuning-cache <file> (the
+// autotuner's decisions for the "auto" tunings).
 #pragma once
 
 #include <cstdint>
@@ -40,6 +41,7 @@ class RunParams {
   const std::string& getOutputDirName() const { return outdir; }
   const std::string& getOutputFilePrefix() const { return outfile_prefix; }
   const std::string& getJsonFile() const { return json_file; }
+  const std::string& getTuningCache() const { return tuning_cache; }
   InputMode getInputMode() const { return input_mode; }
   std::uint64_t getSeed() const { return seed; }
   std::int64_t getTsteps() const { return tsteps; }
@@ -79,6 +81,7 @@ class RunParams {
   std::string outdir = ".";
   std::string outfile_prefix = "RAJAPerf";
   std::string json_file;
+  std::string tuning_cache;
   InputMode input_mode = ReferenceInputs;
   std::uint64_t seed = 42;
   std::int64_t tsteps = 0;

@@ -3,6 +3,8 @@
 // rep loop over the C ABI.
 #include "algorithm/Algorithm.hpp"
 
+#include "Autotune.hpp"
+
 #include "stream/Stream.hpp"
 
 namespace rajaperf {
@@ -22,11 +24,7 @@ REDUCE_SUM::REDUCE_SUM(const RunParams& params) : KernelBase(Algorithm_REDUCE_SU
   setComplexity(Complexity::N);
   setVariantDefined(MI355X_Native);
 }
-void REDUCE_SUM::setMI355XTuningDefinitions(VariantID vid) {
-  addVariantTuningName(vid, "default");
-  addVariantTuningName(vid, "grid_1024");
-  addVariantTuningName(vid, "grid_4096");
-}
+void REDUCE_SUM::setMI355XTuningDefinitions(VariantID vid) { stream::setReduceTunings(*this, vid); }
 void REDUCE_SUM::setUp(VariantID, size_t) {
   allocAndInitData(m_x, getActualProblemSize(), -1.0, 1.0);
   m_sum = 0.0;
@@ -37,13 +35,14 @@ void REDUCE_SUM::updateChecksum(VariantID vid, size_t t) {
 void REDUCE_SUM::tearDown(VariantID, size_t) { deallocData(m_x); }
 void REDUCE_SUM::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  rpx_tuning tune = defaultTuning();
-  if (tune_idx == 1) tune.grid_size = 1024;
-  if (tune_idx == 2) tune.grid_size = 4096;
-  const size_t sb = rpx_reduce_scratch_bytes(n, &tune);
+  const size_t sb = stream::reduceScratchBytes(*this, n);
   void* scratch = detail::deviceAlloc(sb + 256);
   rpxCheck(rpx_reduce_scratch_init(scratch, sb, nullptr), "reduce scratch init");
   Real_ptr d_sum = reinterpret_cast<Real_ptr>(static_cast<char*>(scratch) + sb);
+  // Trials write the same sum the timed reps will.
+  const rpx_tuning tune = stream::reduceTuning(*this, tune_idx, [&](const rpx_tuning& t) {
+    rpxCheck(rpx_reduce_sum_f64(m_x, n, d_sum, scratch, sb, &t, nullptr), "REDUCE_SUM");
+  });
   // m_sum = sum every rep (REDUCE_SUM-Seq.cpp:38-45): the last rep's value.
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep)
@@ -73,6 +72,7 @@ void SCAN::setMI355XTuningDefinitions(VariantID vid) {
   addVariantTuningName(vid, "default");  // 64 doubles per work-item
   addVariantTuningName(vid, "items_16");
   addVariantTuningName(vid, "items_32");
+  addVariantTuningName(vid, "auto");
 }
 void SCAN::setUp(VariantID, size_t) {
   allocAndInitDataRandValue(m_x, getActualProblemSize(), -1.0, 1.0);
@@ -87,12 +87,25 @@ void SCAN::tearDown(VariantID, size_t) {
 }
 void SCAN::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
+  const size_t sb = rpx_scan_scratch_bytes(n, nullptr);  // serves every tile size
+  void* scratch = detail::deviceAlloc(sb);
+  rpxCheck(rpx_scan_scratch_init(scratch, sb, nullptr), "scan scratch init");
+  auto with_items = [&](int items) {
+    rpx_tuning t = defaultTuning();
+    t.items_per_thread = items;
+    return t;
+  };
   rpx_tuning tune = defaultTuning();
   if (tune_idx == 1) tune.items_per_thread = 16;
   if (tune_idx == 2) tune.items_per_thread = 32;
-  const size_t sb = rpx_scan_scratch_bytes(n, &tune);
-  void* scratch = detail::deviceAlloc(sb);
-  rpxCheck(rpx_scan_scratch_init(scratch, sb, nullptr), "scan scratch init");
+  if (tune_idx == 3)  // trials write the same y the timed reps will
+    tune = Autotuner::instance()
+               .choose(getName(), "n=" + std::to_string(n),
+                       {{"default", with_items(64)}, {"items_16", with_items(16)}, {"items_32", with_items(32)}},
+                       [&](const rpx_tuning& t) {
+                         rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, &t, nullptr), "SCAN");
+                       })
+               .tuning;
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep)
     rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, &tune, nullptr), "SCAN");

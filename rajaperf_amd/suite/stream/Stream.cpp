@@ -3,6 +3,8 @@
 // rep loop that calls the C ABI (the counterpart of <KERNEL>-Hip.cpp).
 #include "stream/Stream.hpp"
 
+#include "Autotune.hpp"
+
 #include <stdexcept>
 #include <string>
 
@@ -25,17 +27,70 @@ const StreamTune kStreamTunes[] = {
 };
 }  // namespace
 
-void setStreamTunings(KernelBase& k, VariantID vid) {
-  for (const auto& t : kStreamTunes) k.addVariantTuningName(vid, t.name);
+constexpr size_t kNumStreamTunes = sizeof(kStreamTunes) / sizeof(kStreamTunes[0]);
+
+// DOT / REDUCE_SUM: doubles per work-item and grid caps.
+struct ReduceTune {
+  const char* name;
+  int items, grid;
+};
+const ReduceTune kReduceTunes[] = {
+    {"default", 0, 0}, {"grid_1024", 0, 1024}, {"grid_4096", 0, 4096}, {"items_16", 16, 0}, {"items_64", 64, 0},
+};
+constexpr size_t kNumReduceTunes = sizeof(kReduceTunes) / sizeof(kReduceTunes[0]);
+
+rpx_tuning fixedReduceTuning(const KernelBase& k, size_t idx) {
+  rpx_tuning t = k.defaultTuning();
+  if (kReduceTunes[idx].items) t.items_per_thread = kReduceTunes[idx].items;
+  if (kReduceTunes[idx].grid) t.grid_size = kReduceTunes[idx].grid;
+  return t;
 }
 
-rpx_tuning streamTuning(const KernelBase& k, size_t tune_idx) {
+rpx_tuning fixedStreamTuning(const KernelBase& k, size_t idx) {
   rpx_tuning t = k.defaultTuning();
-  const StreamTune& s = kStreamTunes[tune_idx];
+  const StreamTune& s = kStreamTunes[idx];
   if (s.block) t.block_size = s.block;
   if (s.items) t.items_per_thread = s.items;
   t.flags |= s.flags;
   return t;
+}
+
+void setReduceTunings(KernelBase& k, VariantID vid) {
+  for (const auto& t : kReduceTunes) k.addVariantTuningName(vid, t.name);
+  k.addVariantTuningName(vid, "auto");
+}
+
+size_t reduceScratchBytes(const KernelBase& k, Index_type n) {
+  size_t sb = 0;
+  for (size_t i = 0; i < kNumReduceTunes; ++i) {
+    const rpx_tuning t = fixedReduceTuning(k, i);
+    const size_t b = rpx_reduce_scratch_bytes(n, &t);
+    if (b > sb) sb = b;
+  }
+  return sb;
+}
+
+rpx_tuning reduceTuning(const KernelBase& k, size_t tune_idx, const std::function<void(const rpx_tuning&)>& trial) {
+  if (tune_idx < kNumReduceTunes) return fixedReduceTuning(k, tune_idx);
+  std::vector<TuneCandidate> cands;
+  for (size_t i = 0; i < kNumReduceTunes; ++i) cands.push_back({kReduceTunes[i].name, fixedReduceTuning(k, i)});
+  return Autotuner::instance()
+      .choose(k.getName(), "n=" + std::to_string(k.getActualProblemSize()), cands, trial)
+      .tuning;
+}
+
+void setStreamTunings(KernelBase& k, VariantID vid) {
+  for (const auto& t : kStreamTunes) k.addVariantTuningName(vid, t.name);
+  k.addVariantTuningName(vid, "auto");
+}
+
+rpx_tuning streamTuning(const KernelBase& k, size_t tune_idx, const std::function<void(const rpx_tuning&)>& trial) {
+  if (tune_idx < kNumStreamTunes) return fixedStreamTuning(k, tune_idx);
+  std::vector<TuneCandidate> cands;
+  for (size_t i = 0; i < kNumStreamTunes; ++i) cands.push_back({kStreamTunes[i].name, fixedStreamTuning(k, i)});
+  return Autotuner::instance()
+      .choose(k.getName(), "n=" + std::to_string(k.getActualProblemSize()), cands, trial)
+      .tuning;
 }
 
 // ---- COPY (src/stream/COPY.cpp:22-78) ------------------------------------------
@@ -65,7 +120,9 @@ void COPY::tearDown(VariantID, size_t) {
 }
 void COPY::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  const rpx_tuning tune = streamTuning(*this, tune_idx);
+  const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
+    rpxCheck(rpx_copy_f64(m_c, m_a, n, &t, nullptr), "COPY");
+  });
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep) rpxCheck(rpx_copy_f64(m_c, m_a, n, &tune, nullptr), "COPY");
   stopTimer();
@@ -99,7 +156,9 @@ void MUL::tearDown(VariantID, size_t) {
 }
 void MUL::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  const rpx_tuning tune = streamTuning(*this, tune_idx);
+  const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
+    rpxCheck(rpx_mul_f64(m_b, m_c, m_alpha, n, &t, nullptr), "MUL");
+  });
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep)
     rpxCheck(rpx_mul_f64(m_b, m_c, m_alpha, n, &tune, nullptr), "MUL");
@@ -135,9 +194,12 @@ void ADD::tearDown(VariantID, size_t) {
 }
 void ADD::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  const rpx_tuning tune = streamTuning(*this, tune_idx);
+  const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
+    rpxCheck(rpx_add_f64(m_c, m_a, m_b, n, &t, nullptr), "ADD");
+  });
   startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep) rpxCheck(rpx_add_f64(m_c, m_a, m_b, n, &tune, nullptr), "ADD");
+  for (RepIndex_type irep = 0; irep < reps; ++irep)
+    rpxCheck(rpx_add_f64(m_c, m_a, m_b, n, &tune, nullptr), "ADD");
   stopTimer();
 }
 
@@ -172,7 +234,9 @@ void TRIAD::tearDown(VariantID, size_t) {
 }
 void TRIAD::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  const rpx_tuning tune = streamTuning(*this, tune_idx);
+  const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
+    rpxCheck(rpx_triad_f64(m_a, m_b, m_c, m_alpha, n, &t, nullptr), "TRIAD");
+  });
   startTimer();
   for (RepIndex_type irep = 0; irep < reps; ++irep)
     rpxCheck(rpx_triad_f64(m_a, m_b, m_c, m_alpha, n, &tune, nullptr), "TRIAD");
@@ -193,11 +257,7 @@ DOT::DOT(const RunParams& params) : KernelBase(Stream_DOT, params) {
   setComplexity(Complexity::N);
   setVariantDefined(MI355X_Native);
 }
-void DOT::setMI355XTuningDefinitions(VariantID vid) {
-  addVariantTuningName(vid, "default");
-  addVariantTuningName(vid, "grid_1024");
-  addVariantTuningName(vid, "grid_4096");
-}
+void DOT::setMI355XTuningDefinitions(VariantID vid) { setReduceTunings(*this, vid); }
 void DOT::setUp(VariantID, size_t) {
   allocAndInitData(m_a, getActualProblemSize(), 0.0, 1.0);
   allocAndInitData(m_b, getActualProblemSize(), 0.0, 1.0);
@@ -210,14 +270,17 @@ void DOT::tearDown(VariantID, size_t) {
 }
 void DOT::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps(), n = getActualProblemSize();
-  rpx_tuning tune = defaultTuning();
-  if (tune_idx == 1) tune.grid_size = 1024;
-  if (tune_idx == 2) tune.grid_size = 4096;
-  const size_t sb = rpx_reduce_scratch_bytes(n, &tune);
-  void* scratch = detail::deviceAlloc(sb + 256);
+  // Scratch sized for the largest chunk count of any tuning (the smallest
+  // chunks); a trial's result goes to its own scalar.
+  const size_t sb = reduceScratchBytes(*this, n);
+  void* scratch = detail::deviceAlloc(sb + 512);
   rpxCheck(rpx_reduce_scratch_init(scratch, sb, nullptr), "reduce scratch init");
   Real_ptr d_dot = reinterpret_cast<Real_ptr>(static_cast<char*>(scratch) + sb);
+  Real_ptr d_trial = d_dot + 32;
   rpxCheck(rpx_fill_const_f64(d_dot, 1, 0.0, nullptr), "DOT init");
+  const rpx_tuning tune = reduceTuning(*this, tune_idx, [&](const rpx_tuning& t) {
+    rpxCheck(rpx_dot_f64(m_a, m_b, n, d_trial, 0, scratch, sb, &t, nullptr), "DOT");
+  });
   // m_dot += dot every rep (DOT-Seq.cpp:39-45), accumulated on the device in
   // rep order and read back once after the timed loop.
   startTimer();

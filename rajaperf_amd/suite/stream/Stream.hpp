@@ -2,6 +2,8 @@
 // MI355X_Native variant.
 #pragma once
 
+#include <functional>
+
 #include "KernelBase.hpp"
 
 namespace rajaperf {
@@ -11,9 +13,16 @@ void rpxCheck(int status, const char* what);
 
 namespace stream {
 
-// Tunings shared by the element-wise kernels.
+// Tunings shared by the element-wise kernels; the last one, "auto", asks
+// the autotuner, which times the others with `trial` (one rep).
 void setStreamTunings(KernelBase& k, VariantID vid);
-rpx_tuning streamTuning(const KernelBase& k, size_t tune_idx);
+rpx_tuning streamTuning(const KernelBase& k, size_t tune_idx, const std::function<void(const rpx_tuning&)>& trial);
+
+// Tunings of DOT and REDUCE_SUM (chunk sizes and grid caps, then "auto"),
+// and the scratch size that serves all of them.
+void setReduceTunings(KernelBase& k, VariantID vid);
+size_t reduceScratchBytes(const KernelBase& k, Index_type n);
+rpx_tuning reduceTuning(const KernelBase& k, size_t tune_idx, const std::function<void(const rpx_tuning&)>& trial);
 
 class COPY : public KernelBase {
  public:

@@ -63,3 +63,35 @@ def test_harness_all_tunings_run_and_agree(rpx):
         cs = [float(r["checksum"]) for r in runs]
         assert max(cs) - min(cs) < 1e-7, k
         assert all(r["time_total_s"] > 0 for r in runs)
+
+
+def test_harness_autotuner_decides_caches_and_keeps_checksums(rpx, orc, tmp_path):
+    """The "auto" tuning times its candidates before the timed reps, keeps
+    the checksum of Base_Seq, records one decision per (kernel, shape, arch)
+    in the cache file, and a second run reads the decisions back."""
+    cache = str(tmp_path / "tuning.txt")
+    kernels = ["Stream_TRIAD", "Stream_DOT", "Algorithm_SCAN", "Polybench_JACOBI_2D"]
+    args = ["-k", *kernels, "--size", "16900", "--checkrun", "2", "--input-mode", "seeded", "--tsteps", "4",
+            "-t", "auto", "--tuning-cache", cache]
+    first = run_suite(args)
+    assert first["rc"] == 0, first.get("failures")
+    assert sorted(d["kernel"] for d in first["autotune"]) == sorted(kernels)
+    assert not any(d["from_cache"] for d in first["autotune"])
+    lines = open(cache).read().splitlines()
+    assert len(lines) == len(kernels)
+    second = run_suite(args)
+    assert second["rc"] == 0
+    assert all(d["from_cache"] for d in second["autotune"])
+    assert {d["kernel"]: d["choice"] for d in second["autotune"]} == {d["kernel"]: d["choice"] for d in first["autotune"]}
+    for r in first["runs"]:
+        assert r["tuning"] == "auto"
+        kw = dict(size=16900, checkrun=2, npasses=1, input_mode=1)
+        if r["kernel"] == "Polybench_JACOBI_2D":
+            kw["tsteps"] = 4
+        ref = orc.run(r["kernel"], "Base_Seq", **kw)
+        from fractions import Fraction
+        ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
+        if r["kernel"] in BIT_EXACT:
+            assert r["checksum"] == ref_cs, r["kernel"]
+        else:
+            assert abs(float(r["checksum"] - ref_cs)) <= 1e-12 * max(1.0, abs(float(ref_cs))), r["kernel"]
