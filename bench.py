@@ -44,7 +44,7 @@ N_STREAM = 1 << 28
 N_GRID = 8192
 TSTEPS = 50
 DEPTH = 3                          # JACOBI_2D time steps per fused launch
-ROWS = 8                           # output rows per wavefront tile
+ROWS = 16                          # output rows per wavefront tile
 LAUNCHES = TSTEPS // DEPTH         # 16 launches of DEPTH steps that store no B ...
 LAST_DEPTH = TSTEPS - LAUNCHES * DEPTH  # ... then one of 2 steps that stores the final B
 

@@ -231,8 +231,8 @@ int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
  * reading A once and writing only the final B and the new A, the new A going
  * to `scratch` (an N x N grid) and back, so no launch overwrites values a
  * neighbouring tile still reads.  Ends with the result in A and B, bit-equal
- * to rpx_jacobi2d_f64.  Falls back to the two-sweep path for odd N or
- * pointers that are not 16-byte aligned. */
+ * to rpx_jacobi2d_f64.  Falls back to the two-sweep path for odd N,
+ * N > 2^22 or pointers that are not 16-byte aligned. */
 size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
 /* The building block of rpx_jacobi2d_fused_f64: one launch advancing `depth`
  * time steps (1..3) from Ain into Aout, writing the final B (unless
@@ -241,8 +241,9 @@ size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
  * Aout != Ain, and Aout's boundary equal to Ain's (copy it once with
  * rpx_jacobi2d_copy_boundary_f64; the launch may rewrite boundary columns
  * with those same values); callers ping-pong Ain and Aout between launches.
- * tune->items_per_thread: output rows per wavefront tile, 4, 8 (default),
- * 12, 16 or 32. */
+ * tune->items_per_thread: output rows per wavefront tile, 4, 8, 12,
+ * 16 (default), or 32 at depth 1.  Returns RPX_ERR_UNSUPPORTED for odd N,
+ * N > 2^22 or an unsupported rows/depth pair. */
 int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain,
                                   int64_t N, int depth,
                                   const rpx_tuning* tune, void* stream);

@@ -114,19 +114,6 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
   }
 }
 
-// lane l receives v from lane l-1 (lane 0 keeps its own value).
-__device__ __forceinline__ double from_lower_lane(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x138, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-// lane l receives v from lane l+1 (lane 63 keeps its own value).
-__device__ __forceinline__ double from_upper_lane(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x130, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
 // ---- fused time steps: T steps (2T sweeps) per launch -----------------------
 //
 // One wavefront owns a strip of 128 columns (its lanes hold column pairs) and
@@ -143,91 +130,157 @@ __device__ __forceinline__ double from_upper_lane(double v) {
 // owned rows and columns only, each by exactly one wavefront; the new A goes
 // to a different grid than the one read, so no wavefront reads a value
 // another one has already replaced.
+//
+// Per wavefront and (step, level) the interior path issues 8 FP64 adds, 2
+// multiplies and 4 DPP moves and nothing else: row offsets are scalar (the
+// wavefront index is made uniform with readfirstlane) and the wavefronts
+// that touch the grid boundary take a separate path.  Measured on MI355X at
+// N = 8192, a launch of depth 3 moves the ideal 1.08 GB (read A, write the
+// new A); R = 16 rows per wavefront is the fastest tile.
 constexpr int kStripCols = 128;
 
-template <int R, int T, bool WRITE_B>
-__global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ Aout, double* __restrict__ B,
-                                                           const double* __restrict__ Ain, int64_t N, int strips,
-                                                           int chunks) {
+// lane l receives v from lane l-1; lane 0 receives 0.
+__device__ __forceinline__ double lane_shr_zero(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+// lane l receives v from lane l+1; lane 63 receives 0.
+__device__ __forceinline__ double lane_shl_zero(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+constexpr int kAhead = 6;  // level-0 rows loaded ahead of their use
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: untyped dword access
+constexpr int kStoreNT = 2;              // buffer cache policy: nontemporal (the next launch reads it, not this one)
+
+// The cascade of one wavefront.  EDGE = false is the interior case: the
+// wavefront's R+2H rows and 128 columns lie strictly inside the grid, so no
+// level row is a boundary row and no lane holds a boundary column.  The two
+// edge lanes then take 0 for their missing west/east neighbour; that error
+// moves one column inward per level and stays inside the H halo columns.
+// Interior rows are addressed through buffer resources based at the
+// wavefront's first row and column: each access is one SGPR row offset plus
+// the lane's 16-byte offset.  EDGE = true handles any wavefront: loads
+// outside the grid give 0, and each level row re-reads the boundary values
+// it must keep (B's at odd levels, A's -- unchanged in Ain -- at even ones),
+// as the two-sweep update leaves them.  Level-0 rows stream in kAhead rows
+// ahead of their use; the empty asm statement keeps the compiler from
+// hoisting every load to the top, which would hold all R+4T rows in
+// registers at once.
+template <int R, int T, bool WRITE_B, bool EDGE>
+__device__ __forceinline__ void fused_tile(double* __restrict__ Aout, double* __restrict__ B,
+                                           const double* __restrict__ Ain, int64_t N, int64_t rbase, int64_t j0,
+                                           int rows, int lane) {
   constexpr int H = 2 * T;       // sweeps per launch = halo width
-  constexpr int NR = R + 2 * H;  // level-0 rows held
-  const int lane = threadIdx.x & (kWave - 1);
-  // XCD-banded workgroup order (see logical_tile): each XCD walks a band of
-  // row chunks, so the halo rows shared by vertically adjacent chunks are
-  // re-read from the same L2.
-  const unsigned wave_global =
-      logical_tile(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave);
-  const unsigned total = unsigned(strips) * unsigned(chunks);
-  if (wave_global >= total) return;
-  const int strip = int(wave_global % unsigned(strips));  // a workgroup spans 4 adjacent strips
-  const int chunk = int(wave_global / unsigned(strips));
-  const int64_t j0 = int64_t(strip) * (kStripCols - 2 * H) - H;
+  constexpr int NR = R + 2 * H;  // level-0 rows
+  const int pitch = int(N) * int(sizeof(double));
+  const int64_t origin = rbase * N + j0;
+  const int span = NR * pitch;
+  const int voff = lane * 16;
   const int64_t jl = j0 + 2 * lane;       // first column of this lane's pair
   const bool col_in = jl >= 0 && jl < N;  // N even and jl even: jl+1 < N too
-  const int64_t r0 = 1 + int64_t(chunk) * R;
-  int64_t rend = r0 + R;
-  if (rend > N - 1) rend = N - 1;
-  const int rows = int(rend - r0);
-  const int64_t rbase = r0 - H;  // global row of level-0 index 0
+  const bool own = lane >= T && lane < kWave - T;  // columns [j0+H, j0+128-H)
+  const bool x_out = own && jl >= 1 && jl <= N - 2;
+  const bool y_out = own && jl + 1 >= 1 && jl + 1 <= N - 2;
+
+  auto load_row = [&](int k) -> v2d {
+    if constexpr (!EDGE) {
+      const auto in = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(Ain + origin), 0, span, kRsrcWord3);
+      return __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(in, voff, k * pitch, 0));
+    } else {
+      const int64_t row = rbase + k;
+      v2d v{0.0, 0.0};
+      if (col_in && row >= 0 && row < N && k < rows + 2 * H) v = *reinterpret_cast<const v2d*>(Ain + row * N + jl);
+      return v;
+    }
+  };
 
   v2d a[NR];
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int64_t row = rbase + k;
-    a[k] = v2d{0.0, 0.0};
-    if (col_in && row >= 0 && row < N && k < rows + 2 * H) a[k] = *reinterpret_cast<const v2d*>(Ain + row * N + jl);
-  }
-
-  const bool x_out = col_in && jl >= j0 + H && jl < j0 + kStripCols - H && jl >= 1 && jl <= N - 2;
-  const bool y_out = col_in && jl + 1 >= j0 + H && jl + 1 < j0 + kStripCols - H && jl + 1 >= 1 && jl + 1 <= N - 2;
+  for (int k = 0; k < NR && k <= kAhead + 1; ++k) a[k] = load_row(k);
 
   v2d win[H][3];  // win[L-1]: the last three rows of level L
 #pragma unroll
   for (int q = 2; q < NR; ++q) {
+    asm volatile("" ::: "memory");
+    if (q + kAhead < NR) a[q + kAhead] = load_row(q + kAhead);
 #pragma unroll
     for (int L = 1; L <= H; ++L) {
       if (q < 2 * L) continue;  // level L starts once level L-1 has three rows
       const v2d up = (L == 1) ? a[q - 2] : win[L - 2][0];
       const v2d c = (L == 1) ? a[q - 1] : win[L - 2][1];
       const v2d dn = (L == 1) ? a[q] : win[L - 2][2];
-      const double w = from_lower_lane(c.y);
-      const double e = from_upper_lane(c.x);
+      const double w = lane_shr_zero(c.y);
+      const double e = lane_shl_zero(c.x);
       v2d r;
       r.x = point(c.x, w, c.y, dn.x, up.x);
       r.y = point(c.y, c.x, e, dn.y, up.y);
-      const int64_t g = rbase + q - L;  // global row of this level row
-      if (col_in && g >= 0 && g < N) {
-        if (L % 2 == 1) {  // a B level: B's boundary values
-          if (g == 0 || g == N - 1) {
-            r = *reinterpret_cast<const v2d*>(B + g * N + jl);
-          } else {
-            if (jl == 0) r.x = B[g * N];
-            if (jl + 1 == N - 1) r.y = B[g * N + N - 1];
-          }
-        } else {  // an A level: A's boundary values, held at level 0
-          const v2d a0 = a[q - L];
-          if (g == 0 || g == N - 1) {
-            r = a0;
-          } else {
-            if (jl == 0) r.x = a0.x;
-            if (jl + 1 == N - 1) r.y = a0.y;
-          }
+      const int k = q - L;  // row of this level row, relative to rbase
+      if constexpr (EDGE) {
+        const int64_t g = rbase + k;
+        const double* keep = (L % 2 == 1) ? B : Ain;  // the boundary this level keeps
+        if (g == 0 || g == N - 1) {
+          if (col_in) r = *reinterpret_cast<const v2d*>(keep + g * N + jl);
+        } else if (g > 0 && g < N - 1) {
+          if (jl == 0) r.x = keep[g * N];
+          if (jl + 1 == N - 1) r.y = keep[g * N + N - 1];
         }
       }
       win[L - 1][0] = win[L - 1][1];
       win[L - 1][1] = win[L - 1][2];
       win[L - 1][2] = r;
-      if ((L == H || (WRITE_B && L == H - 1)) && g >= r0 && g < rend) {
-        double* out = (L == H ? Aout : B) + g * N + jl;
-        if (x_out && y_out) {
-          *reinterpret_cast<v2d*>(out) = r;
-        } else {
-          if (x_out) out[0] = r.x;
-          if (y_out) out[1] = r.y;
+      if ((L == H || (WRITE_B && L == H - 1)) && k >= H && k < H + R) {
+        if constexpr (!EDGE) {
+          const auto out =
+              __builtin_amdgcn_make_buffer_rsrc((L == H ? Aout : B) + origin, 0, span, kRsrcWord3);
+          if (own) __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, r), out, voff, k * pitch, kStoreNT);
+        } else if (k < H + rows) {
+          double* o = (L == H ? Aout : B) + (rbase + k) * N + jl;
+          if (x_out && y_out) {
+            *reinterpret_cast<v2d*>(o) = r;
+          } else {
+            if (x_out) o[0] = r.x;
+            if (y_out) o[1] = r.y;
+          }
         }
       }
     }
   }
+}
+
+template <int R, int T, bool WRITE_B>
+__global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ Aout, double* __restrict__ B,
+                                                           const double* __restrict__ Ain, int64_t N, int strips,
+                                                           int chunks) {
+  constexpr int H = 2 * T;
+  constexpr int NR = R + 2 * H;
+  const int lane = threadIdx.x & (kWave - 1);
+  // XCD-banded workgroup order (see logical_tile): each XCD walks a band of
+  // row chunks, so the halo rows shared by vertically adjacent chunks are
+  // re-read from the same L2.
+  // readfirstlane: the wavefront index is uniform, and saying so keeps all
+  // row/column arithmetic below on the scalar unit.
+  const unsigned wave_global = __builtin_amdgcn_readfirstlane(
+      logical_tile(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
+  const unsigned total = unsigned(strips) * unsigned(chunks);
+  if (wave_global >= total) return;
+  const int strip = int(wave_global % unsigned(strips));  // a workgroup spans 4 adjacent strips
+  const int chunk = int(wave_global / unsigned(strips));
+  const int64_t j0 = int64_t(strip) * (kStripCols - 2 * H) - H;
+  const int64_t r0 = 1 + int64_t(chunk) * R;
+  int64_t rend = r0 + R;
+  if (rend > N - 1) rend = N - 1;
+  const int rows = int(rend - r0);
+  const int64_t rbase = r0 - H;  // global row of level-0 index 0
+  const bool interior = rows == R && rbase >= 0 && rbase + NR <= N && j0 >= 1 && j0 + kStripCols <= N - 1;
+  if (interior)
+    fused_tile<R, T, WRITE_B, false>(Aout, B, Ain, N, rbase, j0, rows, lane);
+  else
+    fused_tile<R, T, WRITE_B, true>(Aout, B, Ain, N, rbase, j0, rows, lane);
 }
 
 // Copies the boundary rows and columns of src into dst.
@@ -265,7 +318,9 @@ int fused_rows(double* Aout, double* B, const double* Ain, int64_t N, int rows, 
     case 8: return launch_fused<8, T>(Aout, B, Ain, N, write_b, s);
     case 12: return launch_fused<12, T>(Aout, B, Ain, N, write_b, s);
     case 16: return launch_fused<16, T>(Aout, B, Ain, N, write_b, s);
-    case 32: return launch_fused<32, T>(Aout, B, Ain, N, write_b, s);
+    case 32:  // deeper 32-row tiles would not fit in registers
+      if constexpr (T == 1) return launch_fused<32, 1>(Aout, B, Ain, N, write_b, s);
+      return RPX_ERR_UNSUPPORTED;
     default: return RPX_ERR_UNSUPPORTED;
   }
 }
@@ -283,9 +338,17 @@ int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows
 }
 
 constexpr int kDefaultRows = 16;
-constexpr int kDefaultFusedRows = 8;
+constexpr int kDefaultFusedRows = 16;
 constexpr int kDefaultDepth = 3;  // time steps per fused launch
 constexpr int kMaxDepth = 3;
+bool fused_shape_ok(int rows, int depth) {
+  if (depth < 1 || depth > kMaxDepth) return false;
+  return rows == 4 || rows == 8 || rows == 12 || rows == 16 || (rows == 32 && depth == 1);
+}
+
+// Buffer offsets within a wavefront's rows are 32-bit: (R + 4T) * N * 8
+// bytes must stay below 2^31.
+constexpr int64_t kMaxFusedN = int64_t(1) << 22;
 
 template <int R>
 int launch_vec(double* dst, const double* src, int64_t N, hipStream_t s) {
@@ -329,10 +392,11 @@ size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N) { return N > 0 ? size_t(N) * 
 int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain, int64_t N, int depth,
                                   const rpx_tuning* tune, void* stream) {
   if (N < 0 || !Aout || !B || !Ain || Aout == Ain || Aout == B || B == Ain) return RPX_ERR_INVALID_ARGUMENT;
-  if (N % 2 != 0 || !aligned16(Aout) || !aligned16(B) || !aligned16(Ain)) return RPX_ERR_UNSUPPORTED;
+  if (N % 2 != 0 || N > kMaxFusedN || !aligned16(Aout) || !aligned16(B) || !aligned16(Ain))
+    return RPX_ERR_UNSUPPORTED;
   if (N < 3) return RPX_SUCCESS;
   const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
-  if ((rows != 4 && rows != 8 && rows != 12 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth)
+  if (!fused_shape_ok(rows, depth))
     return RPX_ERR_UNSUPPORTED;
   const bool write_b = (tune_flags(tune) & RPX_TUNE_JACOBI_SKIP_B) == 0;
   return fused_launch(Aout, B, Ain, N, rows, depth, write_b, as_stream(stream));
@@ -350,13 +414,13 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   if (N < 0 || tsteps < 0 || (N > 0 && (!A || !B)) || A == B) return RPX_ERR_INVALID_ARGUMENT;
   if (N < 3 || tsteps == 0) return RPX_SUCCESS;
   hipStream_t s = as_stream(stream);
-  const bool vec = (N % 2 == 0) && aligned16(A) && aligned16(B) && aligned16(scratch);
+  const bool vec = (N % 2 == 0) && N <= kMaxFusedN && aligned16(A) && aligned16(B) && aligned16(scratch);
   if (!vec) return rpx_jacobi2d_f64(A, B, N, tsteps, tune, stream);
   if (!scratch || scratch == A || scratch == B) return RPX_ERR_INVALID_ARGUMENT;
   if (scratch_bytes < rpx_jacobi2d_fused_scratch_bytes(N)) return RPX_ERR_SCRATCH_TOO_SMALL;
   const int rows = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultFusedRows);
   const int depth = tune_or(tune, &rpx_tuning::depth, kDefaultDepth);
-  if ((rows != 4 && rows != 8 && rows != 12 && rows != 16 && rows != 32) || depth < 1 || depth > kMaxDepth)
+  if (!fused_shape_ok(rows, depth))
     return RPX_ERR_UNSUPPORTED;
 
   // Launch plan: depths summing to tsteps, an even number of launches so the

@@ -122,7 +122,7 @@ def test_jacobi_bit_exact(rpx, orc, rng, N, tsteps):
 
 @pytest.mark.parametrize("N", [3, 4, 6, 8, 10, 130, 250, 251, 1026, 2050])
 @pytest.mark.parametrize("tsteps", [1, 2, 3, 5, 6])
-@pytest.mark.parametrize("rows,depth", [(4, 1), (8, 1), (16, 1), (32, 1), (4, 2), (8, 2), (12, 2), (16, 2), (32, 2),
+@pytest.mark.parametrize("rows,depth", [(4, 1), (8, 1), (16, 1), (32, 1), (4, 2), (8, 2), (12, 2), (16, 2),
                                         (4, 3), (8, 3), (12, 3), (16, 3)])
 def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows, depth):
     A0 = rng.random(N * N)
@@ -134,6 +134,19 @@ def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows, depth):
     A, B = orc.jacobi2d(A0, B0, N, tsteps)
     assert np.array_equal(dA.to_array(), A)
     assert np.array_equal(dB.to_array(), B)
+
+
+@pytest.mark.parametrize("rows,depth", [(32, 2), (32, 3), (6, 1), (8, 4), (8, 0)])
+def test_jacobi_fused_unsupported_shapes(rpx, rows, depth):
+    """Tile shapes without a kernel are refused before anything is launched."""
+    N = 130
+    A, B, Q = (rpx.DeviceBuffer(8 * N * N) for _ in range(3))
+    L = rpx.lib()
+    with pytest.raises(rpx.RpxError):
+        L.rpx_jacobi2d_fused_launch_f64(Q.ptr, B.ptr, A.ptr, N, depth, rpx.tuning(0, rows), None)
+    if depth > 0:
+        with pytest.raises(rpx.RpxError):
+            L.rpx_jacobi2d_fused_f64(A.ptr, B.ptr, N, 5, Q.ptr, 8 * N * N, rpx.tuning(0, rows, depth=depth), None)
 
 
 @pytest.mark.parametrize("rows", [4, 8, 16, 32])
