@@ -30,9 +30,9 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxSlices = 8;
-constexpr int kTM = 128, kTN = 128, kTK = 64;  // block tile, K step (bytes)
-constexpr int kPadRow = kTK + 16;              // LDS row stride in bytes
-constexpr int64_t kMaxK = 19000;               // keeps every S_t inside int32
+constexpr int kTM = 128, kTN = 128, kTK = 256;  // block tile, K bytes per stage
+constexpr int kChunks = kTK / 16;               // 16-byte chunks per tile row and stage
+constexpr int64_t kMaxK = 19000;                // keeps every S_t inside int32
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -72,7 +72,10 @@ __device__ __forceinline__ void cut(double x, int* digits) {
   }
 }
 
-// One workgroup per (padded) row of A: row exponent, then the slices.
+// One workgroup per (padded) row of A: row exponent, then the slices.  Loads
+// are unrolled and unconditional wherever a whole group lies inside the row,
+// so each work-item keeps 8 or 16 loads in flight instead of waiting on one
+// load at a time.
 template <int S>
 __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restrict__ A, int64_t ni, int64_t nk,
                                                          int64_t nkp, int64_t nip, signed char* __restrict__ As,
@@ -80,9 +83,20 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   __shared__ double red[4];
   const int64_t i = blockIdx.x;
   const int tid = threadIdx.x;
+  const bool live = i < ni;
+  const double* row = A + (live ? i * nk : 0);
   double m = 0.0;
-  if (i < ni)
-    for (int64_t k = tid; k < nk; k += 256) m = fmax(m, fabs(A[i * nk + k]));
+  if (live) {
+    int64_t k = tid;
+    for (; k + 7 * 256 < nk; k += 8 * 256) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = row[k + u * 256];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) m = fmax(m, fabs(v[u]));
+    }
+    for (; k < nk; k += 256) m = fmax(m, fabs(row[k]));
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
   if ((tid & 63) == 0) red[tid >> 6] = m;
@@ -92,13 +106,17 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   if (tid == 0) ea[i] = e;
   // 16 consecutive k per work-item per step; one 16-byte store per slice.
   for (int64_t k0 = int64_t(tid) * 16; k0 < nkp; k0 += 256 * 16) {
+    double x[16];
+    if (live && k0 + 16 <= nk) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = row[k0 + u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = (live && k0 + u < nk) ? row[k0 + u] : 0.0;
+    }
     int dig[16][S];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int64_t k = k0 + u;
-      const double x = (i < ni && k < nk) ? ldexp(A[i * nk + k], -e) : 0.0;
-      cut<S>(x, dig[u]);
-    }
+    for (int u = 0; u < 16; ++u) cut<S>(ldexp(x[u], -e), dig[u]);
 #pragma unroll
     for (int p = 0; p < S; ++p) {
       v4i w;
@@ -111,7 +129,8 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   }
 }
 
-// Column maxima of B as exponents: atomicMax over K chunks.
+// Column maxima of B as exponents: atomicMax over K chunks, 8 loads in
+// flight per work-item.
 __global__ __launch_bounds__(256) void col_exponent_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
                                                            int64_t kchunk, int* __restrict__ fb) {
   const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -119,7 +138,15 @@ __global__ __launch_bounds__(256) void col_exponent_kernel(const double* __restr
   const int64_t kb = int64_t(blockIdx.y) * kchunk;
   const int64_t ke = (kb + kchunk < nk) ? kb + kchunk : nk;
   double m = 0.0;
-  for (int64_t k = kb; k < ke; ++k) m = fmax(m, fabs(B[k * nj + j]));
+  int64_t k = kb;
+  for (; k + 8 <= ke; k += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = B[(k + u) * nj + j];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = fmax(m, fabs(v[u]));
+  }
+  for (; k < ke; ++k) m = fmax(m, fabs(B[k * nj + j]));
   if (m != 0.0) atomicMax(&fb[j], exponent_of(m));
 }
 
@@ -139,13 +166,24 @@ __global__ __launch_bounds__(256) void split_cols_kernel(const double* __restric
   const int jj = tid & 63;
   const int64_t j = j0 + jj;
   const int f = (j < nj) ? fb[j] : 0;
-  for (int kk = tid >> 6; kk < 64; kk += 4) {
-    const int64_t k = k0 + kk;
-    const double x = (j < nj && k < nk) ? ldexp(B[k * nj + j], -(f == INT32_MIN ? 0 : f)) : 0.0;
-    int dig[S];
-    cut<S>(x, dig);
+  const int sh = -(f == INT32_MIN ? 0 : f);
+  double x[16];
+  if (j0 + 64 <= nj && k0 + 64 <= nk) {
 #pragma unroll
-    for (int q = 0; q < S; ++q) lds[q][jj][kk] = (signed char)dig[q];
+    for (int u = 0; u < 16; ++u) x[u] = B[(k0 + (tid >> 6) + 4 * u) * nj + j];
+  } else {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int64_t k = k0 + (tid >> 6) + 4 * u;
+      x[u] = (j < nj && k < nk) ? B[k * nj + j] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    int dig[S];
+    cut<S>(ldexp(x[u], sh), dig);
+#pragma unroll
+    for (int q = 0; q < S; ++q) lds[q][jj][(tid >> 6) + 4 * u] = (signed char)dig[q];
   }
   __syncthreads();
   // 64 columns x 64 bytes per slice: 16 bytes per work-item.
@@ -161,62 +199,100 @@ __device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned G) {
   return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
 }
 
-// 128x128 output tile per workgroup of four wavefronts (64x64 each, a 4x4
+// Output tile (tm, tn) of hardware workgroup b.  With one workgroup per CU,
+// the grid runs in rounds of 256 workgroups, 32 per XCD (workgroup b runs on
+// XCD b % 8).  When both tile counts are multiples of 16, each round covers
+// a 16 x 16 patch of tiles and each XCD a 4 x 8 part of it, so an XCD's L2
+// holds the 4 A tiles and 8 B tiles its 32 workgroups read at every K step
+// (not 1 A and 32 B tiles, as a row-major patch would).  Otherwise tiles
+// are numbered XCD-major.
+__device__ __forceinline__ void tile_of(unsigned b, unsigned G, int tiles_m, int tiles_n, int64_t& tm, int64_t& tn) {
+  if (tiles_m % 16 == 0 && tiles_n % 16 == 0) {
+    const unsigned x = b % 8, slot = b / 8;  // XCD, order within the XCD
+    const unsigned round = slot / 32, pos = slot % 32;
+    const unsigned pn = unsigned(tiles_n) / 16;  // patches per patch row
+    const unsigned patch_m = round / pn, patch_n = round % pn;
+    tm = int64_t(patch_m) * 16 + (x / 2) * 4 + pos / 8;
+    tn = int64_t(patch_n) * 16 + (x % 2) * 8 + pos % 8;
+    return;
+  }
+  const unsigned t = xcd_major(b, G);
+  tm = t / unsigned(tiles_n);
+  tn = t % unsigned(tiles_n);
+}
+
+// LDS slot (in 16-byte chunks) of chunk c of tile row r.  The XOR spreads
+// the 16 rows an MFMA fragment reads at one k over all 16 bank groups, and
+// the 16 chunks a loader wavefront writes per row likewise.
+__device__ __forceinline__ int swz(int r, int c) { return r * kChunks + (c ^ (r & 15)); }
+
+// 128x128 output tile per workgroup of eight wavefronts (32x64 each, a 2x4
 // grid of 16x16 int32 accumulators).  For t = s-1 down to 0 the K loop runs
 // over the concatenation of the slice pairs (p, t-p); S_t is then folded into
-// FP64 accumulators with its power-of-two weight.
+// FP64 accumulators with its power-of-two weight.  A stage is 256 bytes of K
+// for A and B (32 MFMAs per wavefront); the next stage's global loads are in
+// flight in registers while it computes.  The per-wavefront tile is sized so
+// two wavefronts fit on each SIMD (256 registers each), one issuing while the
+// other waits on LDS or memory.
+constexpr int kThreads = 512;
+constexpr int kWaveM = 32, kWaveN = 64;
+constexpr int kFm = kWaveM / 16, kFn = kWaveN / 16;  // 16x16 tiles per wavefront
+constexpr int kLoads = kTM * kChunks / kThreads;       // 16-byte chunks per work-item per operand
+
 template <int S>
-__global__ __launch_bounds__(256) void ozaki_gemm_kernel(double* __restrict__ C, const signed char* __restrict__ As,
-                                                         const signed char* __restrict__ Bs,
-                                                         const int* __restrict__ ea, const int* __restrict__ fb,
-                                                         double alpha, int64_t ni, int64_t nj, int64_t nip,
-                                                         int64_t njp, int64_t nkp, int tiles_n) {
-  __shared__ __attribute__((aligned(16))) signed char Al[2][kTM * kPadRow];
-  __shared__ __attribute__((aligned(16))) signed char Bl[2][kTN * kPadRow];
-  const unsigned tile = xcd_major(blockIdx.x, gridDim.x);
-  const int64_t m0 = int64_t(tile / unsigned(tiles_n)) * kTM;
-  const int64_t n0 = int64_t(tile % unsigned(tiles_n)) * kTN;
+__global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(double* __restrict__ C, const signed char* __restrict__ As,
+                                                              const signed char* __restrict__ Bs,
+                                                              const int* __restrict__ ea, const int* __restrict__ fb,
+                                                              double alpha, int64_t ni, int64_t nj, int64_t nip,
+                                                              int64_t njp, int64_t nkp, int tiles_n) {
+  __shared__ v4i Al[2][kTM * kChunks];
+  __shared__ v4i Bl[2][kTN * kChunks];
+  int64_t tm, tn;
+  tile_of(blockIdx.x, gridDim.x, int(nip / kTM), tiles_n, tm, tn);
+  const int64_t m0 = tm * kTM, n0 = tn * kTN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int wm = (wave >> 1) * kWaveM, wn = (wave & 1) * kWaveN;
   const int64_t ksteps = nkp / kTK;
+  // Loader mapping: work-item tid moves chunk lc of rows lr + 32u, u < kLoads.
+  constexpr int kRowStep = kThreads / kChunks;
+  const int lc = tid % kChunks, lr = tid / kChunks;
+  const int fr = lane & 15, fk = lane >> 4;  // MFMA operand row and k quarter
 
-  // Global -> register staging: two 16-byte chunks of A and of B per work-item.
-  const int c0 = tid, c1 = tid + 256;
-  const int r0 = c0 >> 2, r1 = c1 >> 2, o0 = (c0 & 3) * 16, o1 = (c1 & 3) * 16;
-
-  v4d cacc[4][4];
+  v4d cacc[kFm][kFn];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < kFm; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) cacc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < kFn; ++j) cacc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
   for (int t = S - 1; t >= 0; --t) {
     const int p_lo = t - (S - 1) > 0 ? t - (S - 1) : 0;
     const int p_hi = t < S - 1 ? t : S - 1;
     const int64_t nsteps = int64_t(p_hi - p_lo + 1) * ksteps;
-    v4i acc[4][4];
+    v4i acc[kFm][kFn];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < kFm; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+      for (int j = 0; j < kFn; ++j) acc[i][j] = v4i{0, 0, 0, 0};
 
-    v4i ga0, ga1, gb0, gb1;
+    v4i ga[kLoads], gb[kLoads];
     auto gload = [&](int64_t st) {
       const int p = p_lo + int(st / ksteps);
       const int q = t - p;
       const int64_t k0 = (st % ksteps) * kTK;
-      const signed char* a = As + (int64_t(p) * nip + m0) * nkp + k0;
-      const signed char* b = Bs + (int64_t(q) * njp + n0) * nkp + k0;
-      ga0 = *reinterpret_cast<const v4i*>(a + int64_t(r0) * nkp + o0);
-      ga1 = *reinterpret_cast<const v4i*>(a + int64_t(r1) * nkp + o1);
-      gb0 = *reinterpret_cast<const v4i*>(b + int64_t(r0) * nkp + o0);
-      gb1 = *reinterpret_cast<const v4i*>(b + int64_t(r1) * nkp + o1);
+      const signed char* a = As + (int64_t(p) * nip + m0 + lr) * nkp + k0 + 16 * lc;
+      const signed char* b = Bs + (int64_t(q) * njp + n0 + lr) * nkp + k0 + 16 * lc;
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        ga[u] = *reinterpret_cast<const v4i*>(a + int64_t(kRowStep * u) * nkp);
+        gb[u] = *reinterpret_cast<const v4i*>(b + int64_t(kRowStep * u) * nkp);
+      }
     };
     auto lstore = [&](int buf) {
-      *reinterpret_cast<v4i*>(&Al[buf][r0 * kPadRow + o0]) = ga0;
-      *reinterpret_cast<v4i*>(&Al[buf][r1 * kPadRow + o1]) = ga1;
-      *reinterpret_cast<v4i*>(&Bl[buf][r0 * kPadRow + o0]) = gb0;
-      *reinterpret_cast<v4i*>(&Bl[buf][r1 * kPadRow + o1]) = gb1;
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        Al[buf][swz(lr + kRowStep * u, lc)] = ga[u];
+        Bl[buf][swz(lr + kRowStep * u, lc)] = gb[u];
+      }
     };
 
     gload(0);
@@ -226,16 +302,20 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(double* __restrict__ C,
     for (int64_t st = 0; st < nsteps; ++st) {
       const int buf = int(st & 1);
       if (st + 1 < nsteps) gload(st + 1);
-      v4i af[4], bf[4];
-      const int fr = lane & 15, fk = (lane >> 4) * 16;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const v4i*>(&Al[buf][(wm + i * 16 + fr) * kPadRow + fk]);
+      for (int kk = 0; kk < kChunks / 4; ++kk) {
+        const int c = kk * 4 + fk;
+        v4i af[kFm], bf[kFn];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const v4i*>(&Bl[buf][(wn + j * 16 + fr) * kPadRow + fk]);
+        for (int i = 0; i < kFm; ++i) af[i] = Al[buf][swz(wm + i * 16 + fr, c)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < kFn; ++j) bf[j] = Bl[buf][swz(wn + j * 16 + fr, c)];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < kFm; ++i)
+#pragma unroll
+          for (int j = 0; j < kFn; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+      }
       if (st + 1 < nsteps) {
         lstore(buf ^ 1);
         __syncthreads();
@@ -243,9 +323,9 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(double* __restrict__ C,
     }
     const double w = ldexp(1.0, -7 * (t + 2));
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < kFm; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < kFn; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cacc[i][j][r] += double(acc[i][j][r]) * w;
   }
@@ -253,9 +333,9 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(double* __restrict__ C,
   // int32 16x16 accumulator layout: element r of lane l is row 4*(l/16)+r,
   // column l%16.
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < kFm; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < kFn; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gi = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
@@ -277,14 +357,14 @@ int run_mixed(double* C, const double* A, const double* B, double alpha, int64_t
   int* fb = reinterpret_cast<int*>(base + l.fb_off);
   split_rows_kernel<S><<<dim3(unsigned(l.nip)), dim3(256), 0, s>>>(A, ni, nk, l.nkp, l.nip, As, ea);
   fill_int_kernel<<<dim3(unsigned(ceil_div(l.njp, 256))), dim3(256), 0, s>>>(fb, l.njp, INT32_MIN);
-  const int64_t kchunk = 256;
+  const int64_t kchunk = 64;
   if (nk > 0)
     col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 256)), unsigned(ceil_div(nk, kchunk))), dim3(256), 0, s>>>(
         B, nk, nj, kchunk, fb);
   split_cols_kernel<S><<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 64)), dim3(256), 0, s>>>(B, nk, nj, l.nkp,
                                                                                               l.njp, fb, Bs);
   const int64_t tm = l.nip / kTM, tn = l.njp / kTN;
-  ozaki_gemm_kernel<S><<<dim3(unsigned(tm * tn)), dim3(256), 0, s>>>(C, As, Bs, ea, fb, alpha, ni, nj, l.nip, l.njp,
+  ozaki_gemm_kernel<S><<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, As, Bs, ea, fb, alpha, ni, nj, l.nip, l.njp,
                                                                     l.nkp, int(tn));
   return launch_status();
 }

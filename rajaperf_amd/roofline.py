@@ -181,28 +181,34 @@ def config4(rows, reps, sample_rows=8):
     ms_f64 = _time(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None), reps)
     naive = abi.tuning(flags=abi.RPX_TUNE_NAIVE)
     ms_naive = _time(lambda: L.rpx_gemm_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, naive, None), 1, warmup=1)
-    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 7)
-    scr = B(sb)
-    ms_mix = _time(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 7, scr.ptr, sb, None),
-                   reps)
     # error on sampled rows against 80-bit host arithmetic
     A = dA.to_array().reshape(n, n)
     Bh = dB.to_array().reshape(n, n)
     rng = np.random.default_rng(0)
     rows_idx = np.sort(rng.choice(n, sample_rows, replace=False))
-    Cmix = Cm.to_array().reshape(n, n)[rows_idx]
-    C64 = Cf.to_array().reshape(n, n)[rows_idx]
     Al = A[rows_idx].astype(np.longdouble)
     ref = (Al @ Bh.astype(np.longdouble)) * np.longdouble(alpha)
     scale = alpha * (np.abs(A[rows_idx]) @ np.abs(Bh))
-    err_mix = float(np.max(np.abs(Cmix - ref) / scale))
+    C64 = Cf.to_array().reshape(n, n)[rows_idx]
     err_f64 = float(np.max(np.abs(C64 - ref) / scale))
     _row(rows, "Polybench_GEMM(fp64-mfma)", 4, ms_f64, None, flops, "mfma", FP64_TFLOPS, err_vs_exact=err_f64)
     _row(rows, "Polybench_GEMM(fp64-naive)", 4, ms_naive, None, flops, "mfma", FP64_TFLOPS)
-    _row(rows, "Polybench_GEMM(mixed-int8x7)", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
-         err_vs_exact=err_mix, int8_tops=round(28 * flops / (ms_mix / 1e3) / 1e12, 1),
-         speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
-    del dA, dB, Cf, Cm, scr
+    # s slices: s(s+1)/2 int8 slice products; the error bound is (s+3) 2^-7s
+    # relative to |alpha|(|A||B|)_ij (7 slices: 1.8e-14, 6 slices: 2.0e-12).
+    for s_ in (7, 6):
+        sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, s_)
+        scr = B(sb)
+        ms_mix = _time(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, s_, scr.ptr, sb,
+                                                    None), reps)
+        Cmix = Cm.to_array().reshape(n, n)[rows_idx]
+        err_mix = float(np.max(np.abs(Cmix - ref) / scale))
+        pairs = s_ * (s_ + 1) // 2
+        _row(rows, f"Polybench_GEMM(mixed-int8x{s_})", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
+             err_vs_exact=err_mix, err_bound=(s_ + 3) * 2.0 ** (-7 * s_), slice_products=pairs,
+             int8_tops=round(pairs * flops / (ms_mix / 1e3) / 1e12, 1),
+             speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
+        del scr
+    del dA, dB, Cf, Cm
     mass_mixed(rows, reps)
 
 
