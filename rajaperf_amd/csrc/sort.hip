@@ -9,13 +9,15 @@
 // non-negative values, all bits flipped for negative ones), so unsigned order
 // is numeric order with -0.0 before +0.0.  Every segment is cut into tiles of
 // 4096 keys; each pass is three launches over all tiles of all segments:
-//   hist:    per-tile digit counts (LDS atomics) -> counts[seg][digit][tile]
+//   hist:    per-tile digit counts (LDS atomics) -> counts[seg][tile][digit]
 //   offsets: one workgroup per segment scans its counts digit-major, giving
-//            every (digit, tile) its output offset inside the segment
+//            every (digit, tile) its output offset inside the segment (the
+//            tile-major layout makes each step's 256 loads one coalesced row)
 //   scatter: per tile, a stable rank of each key among equal digits (64-lane
 //            ballot matching row by row within a wavefront, then a prefix
-//            over the wavefronts), and the key is written to
-//            offset + rank.
+//            over the wavefronts); the tile is first reordered by digit in
+//            LDS, then written out so that the keys of one digit go to
+//            consecutive addresses from consecutive work-items.
 // A wavefront owns 1024 consecutive keys of its tile and reads them as 16
 // coalesced rows of 64, which keeps the ranking order equal to the tile order
 // (LSD stability).  Pass 0 converts doubles to keys on load; pass 7 converts
@@ -80,28 +82,53 @@ __device__ __forceinline__ u64 from_key(u64 u) { return (u >> 63) ? (u & 0x7FFFF
 
 __device__ __forceinline__ int tile_pos(int wave, int r, int lane) { return wave * (kRows * kWave) + r * kWave + lane; }
 
+// Per-wavefront sub-histograms: a digit shared by many keys (the top byte
+// of doubles of one sign and binade) then contends within one wavefront only.
 __global__ __launch_bounds__(kBS) void hist_kernel(const u64* __restrict__ src, unsigned* __restrict__ counts, Plan plan,
                                                    int shift, int convert, int src_is_x) {
-  __shared__ unsigned h[kRadix];
+  __shared__ unsigned h[kBS / kWave][kRadix];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const TileInfo ti = get_tile(plan, blockIdx.x);
   const long long sbase = (src_is_x ? ti.seg_x : ti.seg_alt) + (long long)ti.local * kTile;
-  h[tid] = 0u;
+#pragma unroll
+  for (int w = 0; w < kBS / kWave; ++w) h[w][tid] = 0u;
   __syncthreads();
+  u64 k[kRows];
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const int p = tile_pos(wave, r, lane);
-    if (p < ti.len) {
-      u64 k = src[sbase + p];
-      if (convert) k = to_key(k);
-      atomicAdd(&h[(k >> shift) & 0xFF], 1u);
+    k[r] = p < ti.len ? src[sbase + p] : 0ull;
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (tile_pos(wave, r, lane) < ti.len) {
+      const u64 key = convert ? to_key(k[r]) : k[r];
+      atomicAdd(&h[wave][(key >> shift) & 0xFF], 1u);
     }
   }
   __syncthreads();
-  counts[ti.count_base + (long long)tid * ti.ntiles + ti.local] = h[tid];
+  counts[ti.count_base + (long long)ti.local * kRadix + tid] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+}
+
+// Exclusive scan over the 256 work-items of a workgroup (one value each).
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* wsum) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  unsigned incl = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const unsigned o = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += o;
+  }
+  if (lane == kWave - 1) wsum[wave] = incl;
+  __syncthreads();
+  unsigned run = incl - v;
+  for (int w = 0; w < wave; ++w) run += wsum[w];
+  return run;
 }
 
 // One workgroup per segment: exclusive scan of counts in digit-major order.
+// Work-item d owns digit d; the tiles are visited in batches of 8 so that 8
+// coalesced loads are in flight at a time.
 __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ counts, Plan plan) {
   __shared__ unsigned wsum[kBS / kWave];
   long long base;
@@ -113,24 +140,32 @@ __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ cou
     base = (long long)blockIdx.x * plan.tiles_per_seg * kRadix;
     nt = plan.tiles_per_seg;
   }
-  const int d = threadIdx.x, lane = d & 63, wave = d >> 6;
-  unsigned* row = counts + base + (long long)d * nt;
+  const int d = threadIdx.x;
+  unsigned* col = counts + base + d;  // col[t * kRadix]: digit d of tile t
   unsigned s = 0;
-  for (int t = 0; t < nt; ++t) s += row[t];
-  // exclusive scan of the 256 row sums
-  unsigned incl = s;
+  int t = 0;
+  for (; t + 8 <= nt; t += 8) {
+    unsigned v[8];
 #pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const unsigned o = __shfl_up(incl, off, kWave);
-    if (lane >= off) incl += o;
+    for (int u = 0; u < 8; ++u) v[u] = col[(long long)(t + u) * kRadix];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
   }
-  if (lane == kWave - 1) wsum[wave] = incl;
-  __syncthreads();
-  unsigned run = incl - s;
-  for (int w = 0; w < wave; ++w) run += wsum[w];
-  for (int t = 0; t < nt; ++t) {
-    const unsigned c = row[t];
-    row[t] = run;
+  for (; t < nt; ++t) s += col[(long long)t * kRadix];
+  unsigned run = block_exclusive_scan(s, wsum);
+  for (t = 0; t + 8 <= nt; t += 8) {
+    unsigned v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = col[(long long)(t + u) * kRadix];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      col[(long long)(t + u) * kRadix] = run;
+      run += v[u];
+    }
+  }
+  for (; t < nt; ++t) {
+    const unsigned c = col[(long long)t * kRadix];
+    col[(long long)t * kRadix] = run;
     run += c;
   }
 }
@@ -140,13 +175,16 @@ __global__ __launch_bounds__(kBS) void scatter_kernel(const u64* __restrict__ sr
                                                       int convert_in, int convert_out, int src_is_x) {
   __shared__ unsigned cnt[kBS / kWave][kRadix];
   __shared__ unsigned goff[kRadix];
+  __shared__ unsigned dstart[kRadix];  // first LDS slot of each digit in the reordered tile
+  __shared__ unsigned wsum[kBS / kWave];
+  __shared__ u64 sorted[kTile];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const TileInfo ti = get_tile(plan, blockIdx.x);
   const long long sbase = (src_is_x ? ti.seg_x : ti.seg_alt) + (long long)ti.local * kTile;
   const long long dbase = src_is_x ? ti.seg_alt : ti.seg_x;
 #pragma unroll
   for (int w = 0; w < kBS / kWave; ++w) cnt[w][tid] = 0u;
-  goff[tid] = counts[ti.count_base + (long long)tid * ti.ntiles + ti.local];
+  goff[tid] = counts[ti.count_base + (long long)ti.local * kRadix + tid];
   __syncthreads();
 
   u64 key[kRows];
@@ -176,26 +214,33 @@ __global__ __launch_bounds__(kBS) void scatter_kernel(const u64* __restrict__ sr
     if (valid && (m >> lane) == 1ull) cnt[wave][d] = base + unsigned(__popcll(m));
   }
   __syncthreads();
-  // Prefix over wavefronts for every digit (thread = digit).
-  {
-    unsigned run = 0;
+  // Prefix over wavefronts for every digit (thread = digit), then over the
+  // digits: each key's slot in the tile reordered by digit.
+  unsigned total = 0;
 #pragma unroll
-    for (int w = 0; w < kBS / kWave; ++w) {
-      const unsigned c = cnt[w][tid];
-      cnt[w][tid] = run;
-      run += c;
-    }
+  for (int w = 0; w < kBS / kWave; ++w) {
+    const unsigned c = cnt[w][tid];
+    cnt[w][tid] = total;
+    total += c;
   }
+  dstart[tid] = block_exclusive_scan(total, wsum);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const int p = tile_pos(wave, r, lane);
     if (p < ti.len) {
-      const u64 k = key[r];
-      const unsigned d = unsigned(k >> shift) & 0xFFu;
-      const long long pos = (long long)goff[d] + cnt[wave][d] + rank[r];
-      if (pos < ti.seg_len) dst[dbase + pos] = convert_out ? from_key(k) : k;
+      const unsigned d = unsigned(key[r] >> shift) & 0xFFu;
+      sorted[dstart[d] + cnt[wave][d] + rank[r]] = key[r];
     }
+  }
+  __syncthreads();
+  // Slot i holds the (i - dstart[d])-th key of digit d in the tile, which
+  // goes to goff[d] + i - dstart[d]: runs of one digit are contiguous.
+  for (int i = tid; i < ti.len; i += kBS) {
+    const u64 k = sorted[i];
+    const unsigned d = unsigned(k >> shift) & 0xFFu;
+    const long long pos = (long long)goff[d] + (i - dstart[d]);
+    if (pos < ti.seg_len) dst[dbase + pos] = convert_out ? from_key(k) : k;
   }
 }
 
