@@ -68,6 +68,9 @@ typedef struct rpx_tuning {
 #define RPX_TUNE_JACOBI_SKIP_B     0x4 /* rpx_jacobi2d_fused_launch_f64: leave B's
                                            interior unwritten, for a launch whose B a
                                            later launch rewrites before it is read */
+#define RPX_TUNE_MASS_SINGLE_LAUNCH 0x8 /* rpx_mass3dpa_batched_run: one persistent
+                                           launch for all orders instead of one per
+                                           order present */
 
 /* ---- library / runtime support ---------------------------------------- *
  * Counterparts of the suite's HIP data utilities
@@ -288,17 +291,20 @@ int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D,
                      const double* X, double* Y, int64_t NE, int d1d,
                      int q1d, const rpx_tuning* tune, void* stream);
 
-/* Ragged batches in one persistent launch (the suite runs one fixed order
- * and size per kernel object; MASS3DPA.cpp:26-31).  Batch i is one operator
+/* Ragged batches from one chunk table (the suite runs one fixed order and
+ * size per kernel object; MASS3DPA.cpp:26-31).  Batch i is one operator
  * application, as rpx_mass3dpa_f64 with its own bases, data, element count
  * and order; orders may differ between batches (1 <= d1d <= 8, q1d =
  * d1d + 1).  Results are bit-identical to calling rpx_mass3dpa_f64 per batch.
  * rpx_mass3dpa_batched_plan uploads the batch and chunk tables into
  * `scratch` and waits for that copy (a host synchronisation of `stream`);
- * rpx_mass3dpa_batched_run then launches the persistent kernel on a planned
- * scratch any number of times (no host synchronisation; it may be captured
- * in a graph).  rpx_mass3dpa_batched_f64 does both.  tune->grid_size sets
- * the number of persistent workgroups (default 3 per CU). */
+ * rpx_mass3dpa_batched_run then runs a planned scratch any number of times
+ * (no host synchronisation; it may be captured in a graph): one persistent
+ * launch per order (eight launches; an order without batches exits at once),
+ * or, with RPX_TUNE_MASS_SINGLE_LAUNCH, one persistent launch for all
+ * orders.  rpx_mass3dpa_batched_f64 does both.  tune->grid_size sets the
+ * number of persistent workgroups per launch (default: as many as are
+ * resident at once; 3 per CU for the single launch). */
 typedef struct rpx_mass_batch {
   const double* B;
   const double* Bt;

@@ -31,6 +31,7 @@ RPX_ERR_WAIT_EXPIRED = -1005
 RPX_TUNE_CACHED = 0x1
 RPX_TUNE_NAIVE = 0x2
 RPX_TUNE_JACOBI_SKIP_B = 0x4
+RPX_TUNE_MASS_SINGLE_LAUNCH = 0x8
 
 
 class ExtensionMissing(RuntimeError):

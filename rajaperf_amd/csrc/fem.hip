@@ -46,6 +46,11 @@ constexpr int kLdsBudget = 48 * 1024;
 // registers: re-reading cost 27% at (4,5).
 __device__ __forceinline__ void reload_basis() { asm volatile("" ::: "memory"); }
 
+// Global-memory pointers: with the address space in the type, accesses are
+// global (not FLAT) instructions even where the pointer comes from memory.
+typedef const __attribute__((address_space(1))) double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+
 template <int D, int Q>
 struct MassShape {
   static constexpr int kA = Q * Q * Q;  // DDQ / QQQ / QDD per element
@@ -64,9 +69,9 @@ struct MassShape {
 // last LDS reads still pending: a caller that reuses the buffers must
 // synchronise first.
 template <int D, int Q>
-__device__ __forceinline__ void mass3dpa_elements(const double* __restrict__ Bg, const double* __restrict__ Btg,
-                                                  const double* __restrict__ Dg, const double* __restrict__ Xg,
-                                                  double* __restrict__ Yg, int64_t e0, int ne,
+__device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcdouble* __restrict__ Btg,
+                                                  gcdouble* __restrict__ Dg, gcdouble* __restrict__ Xg,
+                                                  gdouble* __restrict__ Yg, int64_t e0, int ne,
                                                   double* __restrict__ bufA, double* __restrict__ bufB,
                                                   double* __restrict__ sB, double* __restrict__ sBt) {
   using S = MassShape<D, Q>;
@@ -85,7 +90,7 @@ __device__ __forceinline__ void mass3dpa_elements(const double* __restrict__ Bg,
   for (int w = tid; w < ne * D2; w += kBS) {
     if constexpr (D >= 7) reload_basis();
     const int e = w / D2, r = w % D2, dz = r / D, dy = r % D;
-    const double* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
+    gcdouble* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
     double x[D];
 #pragma unroll
     for (int dx = 0; dx < D; ++dx) x[dx] = xp[dx];
@@ -125,7 +130,7 @@ __device__ __forceinline__ void mass3dpa_elements(const double* __restrict__ Bg,
     double v[D];
 #pragma unroll
     for (int dz = 0; dz < D; ++dz) v[dz] = in[dz * Q2];
-    const double* dp = Dg + (e0 + e) * Q3 + r;
+    gcdouble* dp = Dg + (e0 + e) * Q3 + r;
     double* out = bufA + e * S::kA + r;  // QQQ[qz][qy][qx], stride Q2 in qz
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) {
@@ -180,7 +185,7 @@ __device__ __forceinline__ void mass3dpa_elements(const double* __restrict__ Bg,
     double v[Q];
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) v[qz] = in[qz * D2];
-    double* yp = Yg + (e0 + e) * D3 + r;
+    gdouble* yp = Yg + (e0 + e) * D3 + r;
 #pragma unroll
     for (int dz = 0; dz < D; ++dz) {
       double u = 0.0;
@@ -204,7 +209,8 @@ __global__ __launch_bounds__(kBS) void mass3dpa_kernel(const double* __restrict_
   __shared__ double sBt[D * Q];
   const int64_t e0 = int64_t(blockIdx.x) * S::E;
   const int ne = int((NE - e0) < S::E ? (NE - e0) : S::E);
-  mass3dpa_elements<D, Q>(Bg, Btg, Dg, Xg, Yg, e0, ne, bufA, bufB, sB, sBt);
+  mass3dpa_elements<D, Q>((gcdouble*)Bg, (gcdouble*)Btg, (gcdouble*)Dg, (gcdouble*)Xg, (gdouble*)Yg, e0, ne, bufA, bufB,
+                          sB, sBt);
 }
 
 // ---- ragged batches, one persistent launch ----------------------------------
@@ -234,7 +240,10 @@ __device__ __attribute__((noinline)) void mass_batch_chunk(rpx_mass_batch b, int
   double* bufA = (double*)lds;
   double* bufB = bufA + S::E * S::kA;
   double* sB = bufB + S::E * S::kB;
-  mass3dpa_elements<D, D + 1>(b.B, b.Bt, b.D, b.X, b.Y, e0, ne, bufA, bufB, sB, sB + (D + 1) * D);
+  // The descriptors' pointers are global memory: typed so, the element code
+  // issues global (not FLAT) accesses, which do not hold up its LDS waits.
+  mass3dpa_elements<D, D + 1>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0, ne,
+                              bufA, bufB, sB, sB + (D + 1) * D);
 }
 
 // Scratch of a batched plan: a 256-byte header (the chunk count), the batch
@@ -242,7 +251,10 @@ __device__ __attribute__((noinline)) void mass_batch_chunk(rpx_mass_batch b, int
 struct BatchHeader {
   int64_t nchunks;
   int64_t nbatch;
+  int64_t order_begin[9];  // chunks of order d1d are [order_begin[d1d], order_end[d1d])
+  int64_t order_end[9];
 };
+static_assert(sizeof(BatchHeader) <= 256, "plan header");
 
 __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 __host__ __device__ inline size_t chunk_batch_offset(int64_t nbatch) {
@@ -276,6 +288,55 @@ __global__ __launch_bounds__(kBS, 3) void mass3dpa_batched_kernel(const char* __
     }
     __syncthreads();  // the next chunk rewrites the LDS
   }
+}
+
+// The batches of one order, as one persistent launch over that order's range
+// of the chunk table (the plan sorts batches by order, so the range is
+// contiguous).  Same element code, LDS sizes and register allocation as the
+// single-order kernel; an order with no batches exits at once.
+template <int D>
+__global__ __launch_bounds__(kBS) void mass3dpa_order_kernel(const char* __restrict__ plan) {
+  using S = MassShape<D, D + 1>;
+  __shared__ double bufA[S::E * S::kA];
+  __shared__ double bufB[S::E * S::kB];
+  __shared__ double sB[(D + 1) * D];
+  __shared__ double sBt[D * (D + 1)];
+  const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(plan);
+  const int64_t nchunks = hdr->nchunks, nbatch = hdr->nbatch;
+  const int64_t begin = hdr->order_begin[D], end = hdr->order_end[D];
+  const auto* batches = reinterpret_cast<const rpx_mass_batch*>(plan + 256);
+  const auto* chunk_batch = reinterpret_cast<const int*>(plan + chunk_batch_offset(nbatch));
+  const auto* chunk0 = reinterpret_cast<const int64_t*>(plan + chunk0_offset(nbatch, nchunks));
+  for (int64_t c = begin + blockIdx.x; c < end; c += gridDim.x) {
+    const int bi = chunk_batch[c];
+    const rpx_mass_batch b = batches[bi];
+    const int64_t e0 = (c - chunk0[bi]) * S::E;
+    const int ne = int((b.ne - e0) < S::E ? (b.ne - e0) : S::E);
+    mass3dpa_elements<D, D + 1>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0,
+                                ne, bufA, bufB, sB, sBt);
+    __syncthreads();  // the next chunk rewrites the LDS
+  }
+}
+
+// Workgroups of mass3dpa_order_kernel<D> resident per CU (queried once).
+template <int D>
+int order_blocks_per_cu() {
+  static int cached = 0;
+  if (cached == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mass3dpa_order_kernel<D>, kBS, 0) != hipSuccess || n < 1)
+      n = 1;
+    cached = n;
+  }
+  return cached;
+}
+
+template <int D>
+int launch_order(const char* plan, int cus, int64_t grid_override, hipStream_t s) {
+  const int64_t grid = grid_override > 0 ? grid_override : int64_t(order_blocks_per_cu<D>()) * cus;
+  if (grid > INT32_MAX) return RPX_ERR_INVALID_ARGUMENT;
+  mass3dpa_order_kernel<D><<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(plan);
+  return launch_status();
 }
 
 // Elements per chunk of a batch of order d1d (Q1D = d1d + 1).
@@ -728,7 +789,16 @@ int rpx_mass3dpa_batched_plan(const rpx_mass_batch* batches_host, int nbatch, vo
     c += k;
   }
   chunk0[size_t(nbatch)] = c;
-  const BatchHeader h{l.nchunks, nbatch};
+  BatchHeader h{};
+  h.nchunks = l.nchunks;
+  h.nbatch = nbatch;
+  for (int i : order) {
+    const int d = batches_host[i].d1d;
+    const int64_t first = chunk0[size_t(i)];
+    const int64_t last = first + ceil_div(batches_host[i].ne, batch_chunk_elems(d));
+    if (h.order_end[d] == h.order_begin[d]) h.order_begin[d] = first;  // first batch of this order
+    h.order_end[d] = last;
+  }
   hipStream_t st = as_stream(stream);
   char* base = reinterpret_cast<char*>(scratch);
   RPX_TRY(hipMemcpyAsync(base, &h, sizeof(h), hipMemcpyHostToDevice, st));
@@ -746,16 +816,31 @@ int rpx_mass3dpa_batched_plan(const rpx_mass_batch* batches_host, int nbatch, vo
 
 int rpx_mass3dpa_batched_run(const void* scratch, const rpx_tuning* tune, void* stream) {
   if (!scratch) return RPX_ERR_INVALID_ARGUMENT;
-  // Persistent grid: workgroups per CU times CUs (tune->grid_size overrides);
-  // workgroups beyond the chunk count exit at once.
   int dev = 0, cus = 0;
   RPX_TRY(hipGetDevice(&dev));
   RPX_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t grid = tune_or(tune, &rpx_tuning::grid_size, 3 * cus);
+  const char* plan = reinterpret_cast<const char*>(scratch);
+  hipStream_t s = as_stream(stream);
+  const int64_t grid_override = tune_or(tune, &rpx_tuning::grid_size, 0);
+  if ((tune_flags(tune) & RPX_TUNE_MASS_SINGLE_LAUNCH) == 0) {
+    // One persistent launch per order, highest first (the plan's order).
+    int rc = RPX_SUCCESS;
+    if (rc == RPX_SUCCESS) rc = launch_order<8>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<7>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<6>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<5>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<4>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<3>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<2>(plan, cus, grid_override, s);
+    if (rc == RPX_SUCCESS) rc = launch_order<1>(plan, cus, grid_override, s);
+    return rc;
+  }
+  // One persistent launch for all orders: workgroups per CU times CUs
+  // (tune->grid_size overrides); workgroups beyond the chunk count exit at
+  // once.  The kernel finds the tables from the counts in the plan's header.
+  const int64_t grid = grid_override > 0 ? grid_override : 3 * int64_t(cus);
   if (grid > INT32_MAX) return RPX_ERR_INVALID_ARGUMENT;
-  // The kernel finds the tables from the counts in the plan's header.
-  mass3dpa_batched_kernel<<<dim3(unsigned(grid)), dim3(kBS), 0, as_stream(stream)>>>(
-      reinterpret_cast<const char*>(scratch));
+  mass3dpa_batched_kernel<<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(plan);
   return launch_status();
 }
 

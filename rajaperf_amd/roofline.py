@@ -344,8 +344,12 @@ def mass_batches(rows, reps, nbatch=64):
             b = arr[i]
             L.rpx_mass3dpa_f64(b.B, b.Bt, b.D, b.X, b.Y, b.ne, b.d1d, b.q1d, None, None)
     ms_each = _time(per_batch, reps)
-    _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one persistent launch)", 5, ms, nbytes,
+    one = abi.tuning(flags=abi.RPX_TUNE_MASS_SINGLE_LAUNCH)
+    ms_one = _time(lambda: L.rpx_mass3dpa_batched_run(scr.ptr, one, None), reps)
+    _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one persistent launch per order)", 5, ms, nbytes,
          elements=int(nes.sum()), per_batch_launches_ms=round(ms_each, 5), speedup=round(ms_each / ms, 3))
+    _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one persistent launch for all orders)", 5, ms_one,
+         nbytes, elements=int(nes.sum()), per_batch_launches_ms=round(ms_each, 5), speedup=round(ms_each / ms_one, 3))
 
 
 def main(argv=None):

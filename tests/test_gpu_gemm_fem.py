@@ -124,10 +124,12 @@ def _batches(rpx, rng, specs):
     return out
 
 
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("grid", [0, 1, 7])
-def test_mass3dpa_ragged_batches_one_launch(rpx, orc, rng, grid):
-    """Ragged batches of mixed orders in one persistent launch equal one
-    rpx_mass3dpa_f64 call per batch, bit for bit, and the oracle."""
+def test_mass3dpa_ragged_batches_one_launch(rpx, orc, rng, grid, single):
+    """Ragged batches of mixed orders -- one persistent launch per order, or
+    one for all orders -- equal one rpx_mass3dpa_f64 call per batch, bit for
+    bit, and the oracle.  Two batches share order 8, one batch is empty."""
     specs = [(300, 2), (0, 5), (1, 8), (2049, 4), (77, 7), (513, 3), (40, 6), (9, 1), (130, 8)]
     data = _batches(rpx, rng, specs)
     arr = (rpx.MassBatch * len(data))()
@@ -136,7 +138,8 @@ def test_mass3dpa_ragged_batches_one_launch(rpx, orc, rng, grid):
     L = rpx.lib()
     sb = L.rpx_mass3dpa_batched_scratch_bytes(arr, len(data))
     scr = rpx.DeviceBuffer(sb)
-    L.rpx_mass3dpa_batched_f64(arr, len(data), scr.ptr, sb, rpx.tuning(grid_size=grid), None)
+    flags = rpx.RPX_TUNE_MASS_SINGLE_LAUNCH if single else 0
+    L.rpx_mass3dpa_batched_f64(arr, len(data), scr.ptr, sb, rpx.tuning(grid_size=grid, flags=flags), None)
     for ne, d, host, bufs in data:
         ref_bufs = [dev(rpx, a) for a in host]
         if ne:

@@ -1,7 +1,13 @@
 """GPU: a full pass of every kernel captured in one HIP graph.  Graph
 replays, with eager passes and synchronisations in between, must leave the
 same bits as the same number of eager passes on an identical second set of
-buffers (every kernel here is deterministic)."""
+buffers -- except SCAN.  A tile's decoupled look-back stops at the first
+predecessor that has published its inclusive prefix, and which one that is
+depends on timing, so two runs of the same scan may associate the FP64 sums
+differently.  SCAN outputs are held to that reassociation's bound instead:
+at most (tiles + 64) roundings of partial sums no larger than the prefix of
+|x| (integer-valued inputs, where association cannot matter, are checked
+bit for bit in the scan tests)."""
 
 import numpy as np
 import pytest
@@ -26,7 +32,13 @@ def test_graph_pass_replays_match_eager(rpx, sizes):
         st.synchronize()
         got, want = g_pass.outputs(), e_pass.outputs()
         for k in want:
-            assert np.array_equal(got[k], want[k]), k
+            if k == "scan":
+                x = np.abs(g_pass.sx.to_array())
+                tiles = -(-x.size // 1024)  # at least the number of scan tiles
+                bound = (tiles + 64) * np.finfo(np.float64).eps * np.concatenate([[0.0], np.cumsum(x)[:-1]])
+                assert np.all(np.abs(got[k] - want[k]) <= bound), k
+            else:
+                assert np.array_equal(got[k], want[k]), k
 
 
 def test_null_stream_cannot_be_captured(rpx):
