@@ -51,6 +51,10 @@ __device__ __forceinline__ void reload_basis() { asm volatile("" ::: "memory"); 
 typedef const __attribute__((address_space(1))) double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
 
+// Doubles of D a work-item may hold in registers from the start of the
+// element code to its third stage (measured at (4,5): 0.150 -> 0.137 ms).
+constexpr int kMassPrefetch = 24;
+
 template <int D, int Q>
 struct MassShape {
   static constexpr int kA = Q * Q * Q;  // DDQ / QQQ / QDD per element
@@ -83,6 +87,20 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
     sB[t] = Bg[q + Q * d];
     const int d2 = t / Q, q2 = t % Q;
     sBt[t] = Btg[q2 + D * d2];
+  }
+  constexpr int kIt3 = (S::E * Q2 + kBS - 1) / kBS;  // P3 pencils per work-item
+  constexpr bool kPre = kIt3 * Q <= kMassPrefetch;
+  double dpre[kPre ? kIt3 : 1][Q];
+  if constexpr (kPre) {
+#pragma unroll
+    for (int it = 0; it < kIt3; ++it) {
+      const int w = tid + it * kBS;
+      if (w < ne * Q2) {
+        const int e = w / Q2, r = w % Q2;
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) dpre[it][qz] = Dg[(e0 + e) * Q3 + r + qz * Q2];
+      }
+    }
   }
   __syncthreads();
 
@@ -122,22 +140,30 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
     }
   }
   __syncthreads();
-  // P3: z-contraction and the quadrature-point scaling.
-  for (int w = tid; w < ne * Q2; w += kBS) {
-    if constexpr (D >= 7) reload_basis();
-    const int e = w / Q2, r = w % Q2;  // r = qy*Q + qx
-    const double* in = bufB + e * S::kB + r;  // DQQ[dz][qy][qx], stride Q2 in dz
-    double v[D];
+  // P3: z-contraction and the quadrature-point scaling.  The D values were
+  // loaded before P1 when they fit in a few registers (kPre), so their
+  // latency overlaps the first two stages.
 #pragma unroll
-    for (int dz = 0; dz < D; ++dz) v[dz] = in[dz * Q2];
-    gcdouble* dp = Dg + (e0 + e) * Q3 + r;
-    double* out = bufA + e * S::kA + r;  // QQQ[qz][qy][qx], stride Q2 in qz
+  for (int it = 0; it < kIt3; ++it) {
+    const int w = tid + it * kBS;
+    if (w < ne * Q2) {
+      if constexpr (D >= 7) reload_basis();
+      const int e = w / Q2, r = w % Q2;  // r = qy*Q + qx
+      const double* in = bufB + e * S::kB + r;  // DQQ[dz][qy][qx], stride Q2 in dz
+      double v[D];
 #pragma unroll
-    for (int qz = 0; qz < Q; ++qz) {
-      double u = 0.0;
+      for (int dz = 0; dz < D; ++dz) v[dz] = in[dz * Q2];
+      double* out = bufA + e * S::kA + r;  // QQQ[qz][qy][qx], stride Q2 in qz
 #pragma unroll
-      for (int dz = 0; dz < D; ++dz) u += v[dz] * sB[qz * D + dz];
-      out[qz * Q2] = u * dp[qz * Q2];
+      for (int qz = 0; qz < Q; ++qz) {
+        double u = 0.0;
+#pragma unroll
+        for (int dz = 0; dz < D; ++dz) u += v[dz] * sB[qz * D + dz];
+        if constexpr (kPre)
+          out[qz * Q2] = u * dpre[it][qz];
+        else
+          out[qz * Q2] = u * Dg[(e0 + e) * Q3 + r + qz * Q2];
+      }
     }
   }
   __syncthreads();
