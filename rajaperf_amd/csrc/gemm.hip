@@ -20,6 +20,7 @@ using namespace rpx;
 
 namespace {
 
+typedef double v2d __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // ---- plain FP64 comparator ---------------------------------------------------
@@ -35,9 +36,20 @@ __global__ __launch_bounds__(256) void gemm_naive_kernel(double* __restrict__ C,
 }
 
 // ---- FP64 MFMA ---------------------------------------------------------------
+//
+// 128x128 output tile per workgroup of four wavefronts (64x64 each, a 4x4
+// grid of v_mfma_f64_16x16x4f64 tiles), K in slabs of 16, double-buffered
+// in LDS.  Both operands sit in LDS k-contiguous ([m][k] for A, [n][k] for
+// B, rows padded by two doubles), and in each pair of MFMA steps lane (r, q)
+// of a fragment takes the adjacent k = 8h + 2q and 8h + 2q + 1, so one
+// 16-byte LDS read feeds two MFMAs.  The k order inside a slab is thereby
+// permuted; the result is the same sum in another association.  Interior
+// tiles load full slabs with unconditional 16-byte loads (A rows as they
+// are, B as 2x4 blocks transposed in registers); edge tiles, partial slabs
+// and odd shapes take guarded 8-byte loads.
 constexpr int kTM = 128, kTN = 128, kTK = 16;
 constexpr int kThreads = 256;
-constexpr int kLdsStride = kTM + 2;  // [k][m] layout, padded
+constexpr int kLdsRow = kTK + 2;  // doubles per LDS row of either operand
 
 __device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned G) {
   return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
@@ -47,9 +59,9 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
                                                                  const double* __restrict__ A,
                                                                  const double* __restrict__ B, double alpha,
                                                                  int64_t ni, int64_t nj, int64_t nk,
-                                                                 int tiles_n) {
-  __shared__ double As[2][kTK * kLdsStride];  // As[k][m]
-  __shared__ double Bs[2][kTK * kLdsStride];  // Bs[k][n]
+                                                                 int tiles_n, int vec) {
+  __shared__ __attribute__((aligned(16))) double As[2][kTM * kLdsRow];  // As[m][k]
+  __shared__ __attribute__((aligned(16))) double Bs[2][kTN * kLdsRow];  // Bs[n][k]
 
   const unsigned tile = xcd_major(blockIdx.x, gridDim.x);
   const int64_t m0 = int64_t(tile / unsigned(tiles_n)) * kTM;
@@ -59,31 +71,56 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 64;  // wave's row offset in the tile
   const int wn = (wave & 1) * 64;   // wave's column offset
+  const bool interior = vec && m0 + kTM <= ni && n0 + kTN <= nj;
 
-  // Global load assignment: A slab 128 x 16 (row-major in A), B slab 16 x 128.
-  const int a_row = tid >> 1, a_k = (tid & 1) * 8;
-  const int b_k = tid >> 4, b_col = (tid & 15) * 8;
+  // Loader mapping.  A: row a_m, k a_k .. a_k+7.  B: k rows b_k, b_k+1,
+  // columns b_n .. b_n+3.
+  const int a_m = tid >> 1, a_k = (tid & 1) * 8;
+  const int b_k = (tid >> 5) * 2, b_n = (tid & 31) * 4;
 
-  double ra[8], rb[8];
+  double ra[8], rb[8];  // rb[c*2 + t]: B(k = b_k + t, n = b_n + c)
   auto load_slab = [&](int64_t k0) {
-    const int64_t gi = m0 + a_row;
+    if (interior && k0 + kTK <= nk) {
+      const double* pa = A + (m0 + a_m) * nk + k0 + a_k;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t gk = k0 + a_k + u;
-      ra[u] = (gi < ni && gk < nk) ? alpha * A[gi * nk + gk] : 0.0;
-    }
-    const int64_t gk = k0 + b_k;
+      for (int u = 0; u < 4; ++u) {
+        const v2d x = *reinterpret_cast<const v2d*>(pa + 2 * u);
+        ra[2 * u] = alpha * x.x;
+        ra[2 * u + 1] = alpha * x.y;
+      }
+      const double* pb = B + (k0 + b_k) * nj + n0 + b_n;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t gj = n0 + b_col + u;
-      rb[u] = (gk < nk && gj < nj) ? B[gk * nj + gj] : 0.0;
+      for (int t = 0; t < 2; ++t) {
+        const v2d x0 = *reinterpret_cast<const v2d*>(pb + t * nj);
+        const v2d x1 = *reinterpret_cast<const v2d*>(pb + t * nj + 2);
+        rb[0 * 2 + t] = x0.x;
+        rb[1 * 2 + t] = x0.y;
+        rb[2 * 2 + t] = x1.x;
+        rb[3 * 2 + t] = x1.y;
+      }
+    } else {
+      const int64_t gi = m0 + a_m;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t gk = k0 + a_k + u;
+        ra[u] = (gi < ni && gk < nk) ? alpha * A[gi * nk + gk] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int64_t gk = k0 + b_k + t, gj = n0 + b_n + c;
+          rb[c * 2 + t] = (gk < nk && gj < nj) ? B[gk * nj + gj] : 0.0;
+        }
     }
   };
   auto store_slab = [&](int buf) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) As[buf][(a_k + u) * kLdsStride + a_row] = ra[u];
+    for (int u = 0; u < 4; ++u)
+      *reinterpret_cast<v2d*>(&As[buf][a_m * kLdsRow + a_k + 2 * u]) = v2d{ra[2 * u], ra[2 * u + 1]};
 #pragma unroll
-    for (int u = 0; u < 8; ++u) Bs[buf][b_k * kLdsStride + b_col + u] = rb[u];
+    for (int c = 0; c < 4; ++c)
+      *reinterpret_cast<v2d*>(&Bs[buf][(b_n + c) * kLdsRow + b_k]) = v2d{rb[c * 2], rb[c * 2 + 1]};
   };
 
   v4d acc[4][4];
@@ -96,22 +133,27 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
   load_slab(0);
   store_slab(0);
   __syncthreads();
-  const int fr = lane & 15;   // fragment row (A) / column (B)
-  const int fk = lane >> 4;   // fragment k within a 4-deep step
+  const int fr = lane & 15;  // fragment row (A) / column (B)
+  const int fq = lane >> 4;  // k pair within an 8-deep half slab
   for (int64_t s = 0; s < nslabs; ++s) {
     const int buf = int(s & 1);
     if (s + 1 < nslabs) load_slab((s + 1) * kTK);
 #pragma unroll
-    for (int kk = 0; kk < kTK; kk += 4) {
-      double af[4], bf[4];
+    for (int h = 0; h < kTK / 8; ++h) {
+      v2d af[4], bf[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[buf][(kk + fk) * kLdsStride + wm + i * 16 + fr];
+      for (int i = 0; i < 4; ++i)
+        af[i] = *reinterpret_cast<const v2d*>(&As[buf][(wm + i * 16 + fr) * kLdsRow + 8 * h + 2 * fq]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][(kk + fk) * kLdsStride + wn + j * 16 + fr];
+      for (int j = 0; j < 4; ++j)
+        bf[j] = *reinterpret_cast<const v2d*>(&Bs[buf][(wn + j * 16 + fr) * kLdsRow + 8 * h + 2 * fq]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].x, bf[j].x, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].y, bf[j].y, acc[i][j], 0, 0, 0);
+        }
     }
     if (s + 1 < nslabs) {
       store_slab(buf ^ 1);
@@ -151,7 +193,9 @@ int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha, doub
   }
   const int64_t tm = ceil_div(ni, kTM), tn = ceil_div(nj, kTN);
   if (tm * tn > INT32_MAX || tn > INT32_MAX) return RPX_ERR_UNSUPPORTED;
-  gemm_mfma_f64_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, A, B, alpha, ni, nj, nk, int(tn));
+  const int vec = (nk % 2 == 0) && (nj % 2 == 0) && aligned16(A) && aligned16(B);
+  gemm_mfma_f64_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, A, B, alpha, ni, nj, nk, int(tn),
+                                                                           vec);
   return launch_status();
 }
 
