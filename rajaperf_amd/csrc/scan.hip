@@ -46,6 +46,7 @@ struct ScanHeader {
 constexpr size_t kHeaderBytes = 256;
 constexpr int kBS = 256;
 constexpr int kDefaultIPT = 64;
+constexpr int kDefaultScanBS = 256;  // work-items per tile (256 or 512)
 constexpr int kMinTile = kBS * 16;  // scratch is sized for the smallest tile
 constexpr unsigned kSpinLimit = 1u << 22;
 
@@ -88,16 +89,16 @@ struct SegTables {
 // carry the epoch read from the header.  SEG = true: tiles come from the
 // segment tables, the look-back stops at the segment's first tile, and the
 // flags were cleared before the launch (epoch 0, no epoch advance).
-template <int ROWS, bool NT, bool SEG>  // 16-byte rows per lane: a wavefront scans 128*ROWS doubles
-__global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, int64_t n, int64_t ntiles,
+template <int BS, int ROWS, bool NT, bool SEG>  // 16-byte rows per lane: a wavefront scans 128*ROWS doubles
+__global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, int64_t n, int64_t ntiles,
                                                    ScanHeader* __restrict__ hdr,
                                                    unsigned long long* __restrict__ flags,
                                                    double* __restrict__ aggs, double* __restrict__ incls,
                                                    int vec, SegTables seg) {
   constexpr int kRowLen = 2 * kWave;            // doubles per wavefront row
   constexpr int kChunk = kRowLen * ROWS;        // doubles per wavefront
-  constexpr int kTileN = kChunk * (kBS / kWave);
-  __shared__ double wave_tot[kBS / kWave];
+  constexpr int kTileN = kChunk * (BS / kWave);
+  __shared__ double wave_tot[BS / kWave];
   __shared__ unsigned long long s_epoch;
   __shared__ double s_prefix;
 
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(kBS) void scan_kernel(const double* x, double* y, i
   const unsigned long long E = SEG ? 0ull : s_epoch;
   double wave_off = 0.0, tile_agg = 0.0;
 #pragma unroll
-  for (int w = 0; w < kBS / kWave; ++w) {
+  for (int w = 0; w < BS / kWave; ++w) {
     const double wt = wave_tot[w];
     if (w < wave) wave_off += wt;
     tile_agg += wt;
@@ -262,21 +263,23 @@ Layout layout_for(int64_t ntiles) {
   return l;
 }
 
-template <int ROWS, bool SEG>
+template <int ROWS, bool SEG, int BS = kBS>
 void launch_rows(bool nt, unsigned grid, const double* x, double* y, int64_t n, int64_t ntiles, ScanHeader* hdr,
                  unsigned long long* flags, double* aggs, double* incls, int vec, SegTables seg, hipStream_t s) {
   if (nt)
-    scan_kernel<ROWS, true, SEG><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec, seg);
+    scan_kernel<BS, ROWS, true, SEG><<<dim3(grid), dim3(BS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec,
+                                                                      seg);
   else
-    scan_kernel<ROWS, false, SEG><<<dim3(grid), dim3(kBS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec,
-                                                                   seg);
+    scan_kernel<BS, ROWS, false, SEG><<<dim3(grid), dim3(BS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec,
+                                                                       seg);
 }
 
-int scan_one(const double* x, double* y, int64_t n, int ipt, bool nt, void* scratch, size_t scratch_bytes,
+int scan_one(const double* x, double* y, int64_t n, int ipt, int bs, bool nt, void* scratch, size_t scratch_bytes,
              hipStream_t s) {
   if (n == 0) return RPX_SUCCESS;
   if (ipt != 16 && ipt != 32 && ipt != 64) return RPX_ERR_INVALID_ARGUMENT;
-  const int64_t ntiles = ceil_div(n, int64_t(kBS) * ipt);
+  if (bs != 256 && bs != 512) return RPX_ERR_INVALID_ARGUMENT;
+  const int64_t ntiles = ceil_div(n, int64_t(bs) * ipt);
   if (ntiles > INT32_MAX) return RPX_ERR_UNSUPPORTED;
   const Layout l = layout_for(ntiles);
   if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
@@ -288,9 +291,15 @@ int scan_one(const double* x, double* y, int64_t n, int ipt, bool nt, void* scra
   const int vec = aligned16(x) && aligned16(y);
   const unsigned grid = unsigned(ntiles);
   const SegTables none{nullptr, nullptr, nullptr};
-  if (ipt == 16) launch_rows<8, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
-  else if (ipt == 32) launch_rows<16, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
-  else launch_rows<32, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+  if (bs == 512) {
+    if (ipt == 16) launch_rows<8, false, 512>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else if (ipt == 32) launch_rows<16, false, 512>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else launch_rows<32, false, 512>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+  } else {
+    if (ipt == 16) launch_rows<8, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else if (ipt == 32) launch_rows<16, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else launch_rows<32, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+  }
   return launch_status();
 }
 
@@ -331,8 +340,9 @@ int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n, void* scratch,
                            const rpx_tuning* tune, void* stream) {
   if (n < 0 || (n > 0 && (!x || !y || !scratch))) return RPX_ERR_INVALID_ARGUMENT;
   const int ipt = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultIPT);
+  const int bs = tune_or(tune, &rpx_tuning::block_size, kDefaultScanBS);
   const bool nt = (tune_flags(tune) & RPX_TUNE_CACHED) == 0;
-  return scan_one(x, y, n, ipt, nt, scratch, scratch_bytes, as_stream(stream));
+  return scan_one(x, y, n, ipt, bs, nt, scratch, scratch_bytes, as_stream(stream));
 }
 
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host, int64_t nseg) {

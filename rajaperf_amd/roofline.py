@@ -98,6 +98,9 @@ def config2(rows, reps):
         t = abi.tuning(0, ipt)
         _row(rows, f"Algorithm_SCAN(ipt {ipt})", 2,
              _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, t, None), reps), 16 * n)
+    t512 = abi.tuning(512, 64)
+    _row(rows, "Algorithm_SCAN(ipt 64, 512 work-items per tile)", 2,
+         _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, t512, None), reps), 16 * n)
     _row(rows, "Algorithm_SCAN(ipt 64, cached accesses)", 2,
          _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, cached, None), reps), 16 * n)
     if L.rpx_scan_scratch_status(sscr.ptr) != 0:

@@ -19,14 +19,15 @@ def scan_scratch(rpx, n):
     return s, sb
 
 
+@pytest.mark.parametrize("bs", [256, 512])
 @pytest.mark.parametrize("ipt", [16, 32, 64])
 @pytest.mark.parametrize("n", [0, 1, 2, 17, 4095, 4096, 4097, 8192, 12289, 1_000_000, 3_000_017])
-def test_scan_matches_oracle(rpx, orc, rng, n, ipt):
+def test_scan_matches_oracle(rpx, orc, rng, n, ipt, bs):
     x = rng.uniform(-1, 1, n)
     dx = dev(rpx, x)
     y = rpx.DeviceBuffer(8 * max(n, 1))
     s, sb = scan_scratch(rpx, n)
-    rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(0, ipt), None)
+    rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(bs, ipt), None)
     got = y.to_array(count=n)
     ref = orc.scan(x)
     if n:
