@@ -147,13 +147,18 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         bf[j] = *reinterpret_cast<const v2d*>(&Bs[buf][(wn + j * 16 + fr) * kLdsRow + 8 * h + 2 * fq]);
+      // All 16 tiles take their first k before any takes its second, so
+      // each accumulator's two MFMAs are 16 instructions apart.
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].x, bf[j].x, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].y, bf[j].y, acc[i][j], 0, 0, 0);
-        }
     }
     if (s + 1 < nslabs) {
       store_slab(buf ^ 1);
