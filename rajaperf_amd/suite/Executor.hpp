@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -34,12 +35,16 @@ class Executor {
   void writeChecksumReport(const std::string& path) const;
   void writeKernelInfoCSV(const std::string& path) const;
   void writeJson(const std::string& path) const;
+  void writeRooflineCSV(const std::string& path) const;
+  void loadProgress();
+  void appendProgress(const KernelBase* k, VariantID vid, size_t t) const;
   bool tuningSelected(const std::string& name) const;
 
   RunParams run_params;
   std::vector<KernelBase*> kernels;
   std::vector<VariantID> variant_ids;
   std::vector<std::string> failures;
+  std::set<std::string> restored;  // "kernel|variant|tuning" restored from the progress file
   int status = 0;
 };
 

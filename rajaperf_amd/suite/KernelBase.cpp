@@ -49,6 +49,15 @@ void KernelBase::execute(VariantID vid, size_t tune_idx) {  // KernelBase.cpp:35
   running_tuning = getUnknownTuningIdx();
 }
 
+void KernelBase::restoreRun(VariantID vid, size_t t, Checksum_type cs, double tot, double mn, double mx,
+                            int nexec) {
+  checksum[vid].at(t) = cs;
+  tot_time[vid].at(t) = tot;
+  min_time[vid].at(t) = mn;
+  max_time[vid].at(t) = mx;
+  num_exec[vid].at(t) = nexec;
+}
+
 void KernelBase::recordExecTime() {  // KernelBase.cpp:379-389
   num_exec[running_variant].at(running_tuning)++;
   min_time[running_variant].at(running_tuning) = std::min(min_time[running_variant].at(running_tuning), last_time);

@@ -84,6 +84,8 @@ class KernelBase {
   Checksum_type getChecksum(VariantID vid, size_t t) const { return checksum[vid].at(t); }
 
   void execute(VariantID vid, size_t tune_idx);
+  // A run recorded by an earlier invocation (restored under --resume).
+  void restoreRun(VariantID vid, size_t t, Checksum_type cs, double tot, double mn, double mx, int nexec);
 
   void synchronize() {
     if (running_variant == MI355X_Native) detail::deviceSynchronize();

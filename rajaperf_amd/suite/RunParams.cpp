@@ -96,6 +96,12 @@ void RunParams::parse(int argc, const char** argv) {
       if (need(i + 1 < argc && !is_option(argv[i + 1]))) json_file = argv[++i];
     } else if (opt == "--tuning-cache") {
       if (need(i + 1 < argc && !is_option(argv[i + 1]))) tuning_cache = argv[++i];
+    } else if (opt == "--progress") {
+      if (need(i + 1 < argc && !is_option(argv[i + 1]))) progress_file = argv[++i];
+    } else if (opt == "--resume") {
+      resume = true;
+    } else if (opt == "--counters") {
+      if (need(i + 1 < argc && !is_option(argv[i + 1]))) counters_file = argv[++i];
     } else if (opt == "--refvar" || opt == "-rv") {
       if (need(i + 1 < argc && !is_option(argv[i + 1]))) reference_variant = argv[++i];
     } else if (opt == "-k" || opt == "--kernels") {
@@ -230,6 +236,8 @@ void RunParams::printHelpMessage(std::ostream& str) const {
       << "  --tsteps <n> (JACOBI_2D), --nk <n> (GEMM), --d1d/--q1d <n> (FEM)\n"
       << "  -od/--outdir <dir>, -of/--outfile <prefix>, --json <file>\n"
       << "  --tuning-cache <file>  (decisions of the \"auto\" tunings)\n"
+      << "  --progress <file.jsonl> [--resume]  one line per finished run; resume skips recorded runs\n"
+      << "  --counters <rocprofv3 counter_collection.csv>  joined into <prefix>-roofline.csv\n"
       << "  --refvar <variant>, --dryrun, --disable-warmup, --show-progress\n"
       << "  --print-kernels, --print-variants, -h/--help\n";
 }

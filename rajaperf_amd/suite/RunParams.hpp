@@ -42,6 +42,12 @@ class RunParams {
   const std::string& getOutputFilePrefix() const { return outfile_prefix; }
   const std::string& getJsonFile() const { return json_file; }
   const std::string& getTuningCache() const { return tuning_cache; }
+  // One JSON line per finished (kernel, variant, tuning) run, appended as it
+  // ends; with --resume, runs already in the file are restored, not rerun.
+  const std::string& getProgressFile() const { return progress_file; }
+  bool getResume() const { return resume; }
+  // rocprofv3 counter collection (CSV) joined into the roofline report.
+  const std::string& getCountersFile() const { return counters_file; }
   InputMode getInputMode() const { return input_mode; }
   std::uint64_t getSeed() const { return seed; }
   std::int64_t getTsteps() const { return tsteps; }
@@ -82,6 +88,9 @@ class RunParams {
   std::string outfile_prefix = "RAJAPerf";
   std::string json_file;
   std::string tuning_cache;
+  std::string progress_file;
+  bool resume = false;
+  std::string counters_file;
   InputMode input_mode = ReferenceInputs;
   std::uint64_t seed = 42;
   std::int64_t tsteps = 0;

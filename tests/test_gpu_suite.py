@@ -95,3 +95,69 @@ def test_harness_autotuner_decides_caches_and_keeps_checksums(rpx, orc, tmp_path
             assert r["checksum"] == ref_cs, r["kernel"]
         else:
             assert abs(float(r["checksum"] - ref_cs)) <= 1e-12 * max(1.0, abs(float(ref_cs))), r["kernel"]
+
+
+def test_harness_progress_jsonl_and_resume(rpx, tmp_path):
+    """--progress appends one JSON line per finished run; --resume restores
+    the recorded runs of the same shape (checksums and times as recorded)
+    and runs only the rest.  A record of another problem size is not
+    restored."""
+    import json
+    prog = str(tmp_path / "progress.jsonl")
+    common = ["--size", "100000", "--checkrun", "2", "--input-mode", "seeded", "-t", "default", "--progress", prog]
+    first = run_suite(["-k", "Stream_TRIAD", "Stream_COPY", *common])
+    assert first["rc"] == 0, first.get("failures")
+    recs = [json.loads(line) for line in open(prog).read().splitlines()]
+    assert sorted(r["kernel"] for r in recs) == ["Stream_COPY", "Stream_TRIAD"]
+    by_kernel = {r["kernel"]: r for r in recs}
+
+    second = run_suite(["-k", "Stream_TRIAD", "Stream_COPY", "Stream_MUL", *common, "--resume"])
+    assert second["rc"] == 0, second.get("failures")
+    recs2 = [json.loads(line) for line in open(prog).read().splitlines()]
+    assert len(recs2) == 3 and recs2[-1]["kernel"] == "Stream_MUL"  # only the new kernel ran
+    runs = {r["kernel"]: r for r in second["runs"]}
+    assert set(runs) == {"Stream_TRIAD", "Stream_COPY", "Stream_MUL"}
+    for k in ("Stream_TRIAD", "Stream_COPY"):
+        for f in ("checksum_hi", "checksum_lo", "time_total_s", "passes"):
+            assert runs[k][f] == by_kernel[k][f], (k, f)
+
+    third_args = ["-k", "Stream_TRIAD", "--size", "50000", "--checkrun", "2", "--input-mode", "seeded", "-t",
+                  "default", "--progress", prog, "--resume"]
+    third = run_suite(third_args)
+    assert third["rc"] == 0
+    recs3 = [json.loads(line) for line in open(prog).read().splitlines()]
+    assert len(recs3) == 4 and recs3[-1]["problem_size"] == 50000  # reran at the new size
+
+
+def test_harness_roofline_csv_joins_counters(rpx, tmp_path):
+    """<prefix>-roofline.csv: per run, the kernel's algorithmic bytes and
+    FLOPs, their rate, the bound they imply and the fraction of it, the
+    checksum difference from the first run; --counters joins each counter's
+    mean value per dispatch over the device kernels behind the suite kernel."""
+    import csv
+    import os
+    cnt = tmp_path / "counter_collection.csv"
+    triad = '"void (anonymous namespace)::stream_vec_kernel<3, 1, 256, true>(double*, double const*, double const*)"'
+    dot = '"void (anonymous namespace)::reduce_kernel<true, 256, 4, true>(double const*, double const*)"'
+    cnt.write_text("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n"
+                   f"1,{triad},FETCH_SIZE,10\n2,{triad},FETCH_SIZE,30\n3,{dot},FETCH_SIZE,7\n"
+                   f"1,{triad},WRITE_SIZE,4\n")
+    n = 100_000
+    res = run_suite(["-k", "Stream_TRIAD", "Stream_COPY", "--size", str(n), "--checkrun", "2", "--input-mode",
+                     "seeded", "--counters", str(cnt)])
+    assert res["rc"] == 0, res.get("failures")
+    rows = list(csv.DictReader(open(os.path.join(res["outdir"], "RAJAPerf-roofline.csv"))))
+    triad_rows = [r for r in rows if r["Kernel"] == "Stream_TRIAD"]
+    copy_rows = [r for r in rows if r["Kernel"] == "Stream_COPY"]
+    assert triad_rows and copy_rows
+    assert float(triad_rows[0]["Checksum diff"]) == 0.0  # the first run is the reference
+    for r in triad_rows:
+        assert float(r["FETCH_SIZE per dispatch"]) == 20.0
+        assert float(r["WRITE_SIZE per dispatch"]) == 4.0
+        assert int(r["Bytes/rep"]) == 24 * n and int(r["FLOPs/rep"]) == 2 * n
+        gbs = int(r["Bytes/rep"]) / float(r["Time/rep (s)"]) / 1e9
+        assert abs(float(r["GB/s"]) - gbs) <= 1e-6 * gbs
+        assert r["Bound"] == "hbm" and abs(float(r["Frac of bound"]) - gbs / 8000.0) <= 1e-6 * gbs
+        assert float(r["Checksum diff"]) == 0.0  # every TRIAD tuning is bit-exact
+    for r in copy_rows:
+        assert r["FETCH_SIZE per dispatch"] == "" and r["WRITE_SIZE per dispatch"] == ""
