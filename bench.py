@@ -193,13 +193,75 @@ def cpu_baseline():
                       f"at N=2^25 and JACOBI_2D {Ng}x{Ng} x {ts} tsteps, seeded inputs"}
 
 
+def host_info() -> dict:
+    """nproc, CPU model, sockets and NUMA nodes of this host."""
+    info = {"nproc": os.cpu_count() or 0}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        txt = open("/proc/cpuinfo").read()
+        models = [l.split(":", 1)[1].strip() for l in txt.splitlines() if l.startswith("model name")]
+        phys = {l.split(":", 1)[1].strip() for l in txt.splitlines() if l.startswith("physical id")}
+        info["model"] = models[0] if models else "unknown"
+        info["sockets"] = len(phys) or 1
+    except OSError:
+        pass
+    try:
+        info["numa_nodes"] = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node")])
+    except OSError:
+        pass
+    return info
+
+
+def cpu_table() -> str:
+    """The oracle's Base_Seq (one thread) and Base_OpenMP on every kernel of
+    the hot path, at bounded sample sizes (about a second of CPU work each),
+    seeded inputs, two reps.  Rates use the kernels' algorithmic bytes (and
+    2MNK FLOPs for GEMM).  SORT has no Base_OpenMP in the suite."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.build()
+    hi = host_info()
+    threads = max(1, min(16, hi.get("affinity", hi["nproc"])))
+    n = 1 << 24
+    runs = [("Stream_COPY", dict(size=n)), ("Stream_MUL", dict(size=n)), ("Stream_ADD", dict(size=n)),
+            ("Stream_TRIAD", dict(size=n)), ("Stream_DOT", dict(size=n)), ("Algorithm_REDUCE_SUM", dict(size=n)),
+            ("Algorithm_SCAN", dict(size=n)), ("Algorithm_SORT", dict(size=1 << 21)),
+            ("Polybench_JACOBI_2D", dict(size=2046 ** 2, tsteps=5)), ("Polybench_GEMM", dict(size=512 * 512, nk=512)),
+            ("Apps_MASS3DPA", dict(size=125 * 16384, d1d=4, q1d=5)),
+            ("Apps_DIFFUSION3DPA", dict(size=125 * 8192, d1d=4, q1d=5))]
+    lines = [f"# host: {hi.get('model', '?')}; nproc {hi['nproc']}, affinity {hi.get('affinity', '?')}, "
+             f"sockets {hi.get('sockets', '?')}, NUMA nodes {hi.get('numa_nodes', '?')}; "
+             f"Base_OpenMP on {threads} threads",
+             "# command: python bench.py --cpu-table   (oracle Base_Seq / Base_OpenMP, seeded inputs, 2 reps)",
+             f"{'kernel':<22} {'problem size':>13} {'variant':<12} {'ms/rep':>10} {'GB/s':>9} {'GFLOP/s':>9}"]
+    for k, kw in runs:
+        for variant, nt in (("Base_Seq", 1), ("Base_OpenMP", threads)):
+            try:
+                r = orc.run(k, variant, input_mode=1, seed=SEED, checkrun=2, nthreads=nt, **kw)
+            except ValueError:
+                continue
+            t = r["time_s"] / r["reps"]
+            flops = 2 * 512 ** 3 if k == "Polybench_GEMM" else r["flops_per_rep"]
+            lines.append(f"{k:<22} {r['problem_size']:>13} {variant:<12} {t * 1e3:>10.3f} "
+                         f"{r['bytes_per_rep'] / t / 1e9:>9.2f} {flops / t / 1e9:>9.2f}")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-table", action="store_true",
+                    help="print the oracle's Base_Seq/Base_OpenMP timings of every kernel (CPU only) and exit")
     args = ap.parse_args()
+    if args.cpu_table:
+        print(cpu_table())
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the kernels are single-device; multi-GPU sharding is a design note).", file=sys.stderr)
