@@ -18,7 +18,8 @@
 // relative to |alpha|(|A||B|)_ij.  There are s(s+1)/2 slice products.
 //
 // Scratch holds the int8 slices (A: [p][i][k], B: [q][j][k], K contiguous,
-// zero-padded to tile multiples) and the per-row/column exponents.
+// zero-padded to tile multiples), the per-row/column exponents and the
+// int32 partial sums S_t ([t][i][j] over the padded grid).
 #include "rpx_common.hpp"
 
 using namespace rpx;
@@ -30,7 +31,7 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxSlices = 8;
-constexpr int kTM = 128, kTN = 128, kTK = 256;  // block tile, K bytes per stage
+constexpr int kTM = 256, kTN = 256, kTK = 128;  // block tile, K bytes per stage
 constexpr int kChunks = kTK / 16;               // 16-byte chunks per tile row and stage
 constexpr int64_t kMaxK = 19000;                // keeps every S_t inside int32
 
@@ -39,7 +40,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 struct MixedLayout {
   int64_t nip, njp, nkp;
-  size_t a_off, b_off, ea_off, fb_off, total;
+  size_t a_off, b_off, ea_off, fb_off, s_off, total;
 };
 MixedLayout mixed_layout(int64_t ni, int64_t nj, int64_t nk, int s) {
   MixedLayout l;
@@ -50,7 +51,8 @@ MixedLayout mixed_layout(int64_t ni, int64_t nj, int64_t nk, int s) {
   l.b_off = align256(size_t(s) * l.nip * l.nkp);
   l.ea_off = l.b_off + align256(size_t(s) * l.njp * l.nkp);
   l.fb_off = l.ea_off + align256(size_t(l.nip) * 4);
-  l.total = l.fb_off + align256(size_t(l.njp) * 4);
+  l.s_off = l.fb_off + align256(size_t(l.njp) * 4);
+  l.total = l.s_off + align256(size_t(s) * l.nip * l.njp * 4);  // S_t, int32, [t][i][j]
   return l;
 }
 
@@ -221,29 +223,36 @@ __device__ __forceinline__ void tile_of(unsigned b, unsigned G, int tiles_m, int
   tn = t % unsigned(tiles_n);
 }
 
-// LDS slot (in 16-byte chunks) of chunk c of tile row r.  The XOR spreads
-// the 16 rows an MFMA fragment reads at one k over all 16 bank groups, and
-// the 16 chunks a loader wavefront writes per row likewise.
-__device__ __forceinline__ int swz(int r, int c) { return r * kChunks + (c ^ (r & 15)); }
+// LDS slot (in 16-byte chunks) of chunk c of tile row r; the XOR stays
+// inside the row's kChunks slots.  It spreads the 16 rows an MFMA fragment
+// reads at one chunk over all 16 bank groups (16-byte groups of a 256-byte
+// bank cycle), and the chunks a loader wavefront writes likewise.
+__device__ __forceinline__ int swz(int r, int c) {
+  if constexpr (kChunks == 16) return r * kChunks + (c ^ (r & 15));
+  else return r * kChunks + (c ^ ((r >> 1) & (kChunks - 1)));
+}
+static_assert(kChunks == 16 || kChunks == 8, "swizzle covers 8 or 16 chunks per row");
 
-// 128x128 output tile per workgroup of eight wavefronts (32x64 each, a 2x4
+// 256x256 output tile per workgroup of eight wavefronts (64x128 each, a 4x8
 // grid of 16x16 int32 accumulators).  For t = s-1 down to 0 the K loop runs
-// over the concatenation of the slice pairs (p, t-p); S_t is then folded into
-// FP64 accumulators with its power-of-two weight.  A stage is 256 bytes of K
-// for A and B (32 MFMAs per wavefront); the next stage's global loads are in
-// flight in registers while it computes.  The per-wavefront tile is sized so
-// two wavefronts fit on each SIMD (256 registers each), one issuing while the
-// other waits on LDS or memory.
+// over the concatenation of the slice pairs (p, t-p), and the exact int32
+// sum S_t of the tile goes to scratch; ozaki_combine_kernel folds the S_t
+// into C.  The tile size is set by L2 bandwidth: a stage brings 128 bytes of
+// K for 256 rows of A and of B (64 KiB) for 8.4 M MACs, 128 MACs per byte,
+// where a 128x128 tile's 64 left the MFMAs waiting on L2.  Keeping only the
+// int32 accumulators in registers (an FP64 fold would take two more per
+// output) is what lets a wavefront own 64x128 outputs at two wavefronts per
+// SIMD.  Two stages of LDS (128 KiB), one workgroup per CU; the next stage's
+// global loads are in flight in registers while one computes.
 constexpr int kThreads = 512;
-constexpr int kWaveM = 32, kWaveN = 64;
+constexpr int kWaveM = 64, kWaveN = 128;
 constexpr int kFm = kWaveM / 16, kFn = kWaveN / 16;  // 16x16 tiles per wavefront
 constexpr int kLoads = kTM * kChunks / kThreads;       // 16-byte chunks per work-item per operand
+static_assert(kTM == kTN, "one loader mapping for both operands");
 
 template <int S>
-__global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(double* __restrict__ C, const signed char* __restrict__ As,
-                                                              const signed char* __restrict__ Bs,
-                                                              const int* __restrict__ ea, const int* __restrict__ fb,
-                                                              double alpha, int64_t ni, int64_t nj, int64_t nip,
+__global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(int* __restrict__ Sout, const signed char* __restrict__ As,
+                                                              const signed char* __restrict__ Bs, int64_t nip,
                                                               int64_t njp, int64_t nkp, int tiles_n) {
   __shared__ v4i Al[2][kTM * kChunks];
   __shared__ v4i Bl[2][kTN * kChunks];
@@ -253,16 +262,10 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(double* __restrict
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * kWaveM, wn = (wave & 1) * kWaveN;
   const int64_t ksteps = nkp / kTK;
-  // Loader mapping: work-item tid moves chunk lc of rows lr + 32u, u < kLoads.
+  // Loader mapping: work-item tid moves chunk lc of rows lr + kRowStep*u, u < kLoads.
   constexpr int kRowStep = kThreads / kChunks;
   const int lc = tid % kChunks, lr = tid / kChunks;
   const int fr = lane & 15, fk = lane >> 4;  // MFMA operand row and k quarter
-
-  v4d cacc[kFm][kFn];
-#pragma unroll
-  for (int i = 0; i < kFm; ++i)
-#pragma unroll
-    for (int j = 0; j < kFn; ++j) cacc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
   for (int t = S - 1; t >= 0; --t) {
     const int p_lo = t - (S - 1) > 0 ? t - (S - 1) : 0;
@@ -321,31 +324,37 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(double* __restrict
         __syncthreads();
       }
     }
-    const double w = ldexp(1.0, -7 * (t + 2));
+    // int32 16x16 accumulator layout: element r of lane l is row 4*(l/16)+r,
+    // column l%16.  The padded S_t grid covers every tile.
+    int* st_out = Sout + int64_t(t) * nip * njp;
 #pragma unroll
     for (int i = 0; i < kFm; ++i)
 #pragma unroll
       for (int j = 0; j < kFn; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) cacc[i][j][r] += double(acc[i][j][r]) * w;
-  }
-
-  // int32 16x16 accumulator layout: element r of lane l is row 4*(l/16)+r,
-  // column l%16.
-#pragma unroll
-  for (int i = 0; i < kFm; ++i)
-#pragma unroll
-    for (int j = 0; j < kFn; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t gi = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-        const int64_t gj = n0 + wn + j * 16 + (lane & 15);
-        if (gi < ni && gj < nj) {
-          const int fj = fb[gj];
-          const double v = ldexp(cacc[i][j][r], ea[gi] + (fj == INT32_MIN ? 0 : fj));
-          C[gi * nj + gj] = alpha * v;
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gi = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+          const int64_t gj = n0 + wn + j * 16 + (lane & 15);
+          st_out[gi * njp + gj] = acc[i][j][r];
         }
-      }
+  }
+}
+
+// C[i][j] = alpha * 2^(e_i+f_j) * sum_t S_t[i][j] 2^{-7(t+2)}, the terms
+// added in FP64 from t = s-1 (the smallest) down to 0.
+template <int S>
+__global__ __launch_bounds__(256) void ozaki_combine_kernel(double* __restrict__ C, const int* __restrict__ Sin,
+                                                            const int* __restrict__ ea, const int* __restrict__ fb,
+                                                            double alpha, int64_t ni, int64_t nj, int64_t nip,
+                                                            int64_t njp) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= ni * nj) return;
+  const int64_t i = idx / nj, j = idx % nj;
+  double v = 0.0;
+#pragma unroll
+  for (int t = S - 1; t >= 0; --t) v += double(Sin[(int64_t(t) * nip + i) * njp + j]) * ldexp(1.0, -7 * (t + 2));
+  const int fj = fb[j];
+  C[idx] = alpha * ldexp(v, ea[i] + (fj == INT32_MIN ? 0 : fj));
 }
 
 template <int S>
@@ -363,9 +372,11 @@ int run_mixed(double* C, const double* A, const double* B, double alpha, int64_t
         B, nk, nj, kchunk, fb);
   split_cols_kernel<S><<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 64)), dim3(256), 0, s>>>(B, nk, nj, l.nkp,
                                                                                               l.njp, fb, Bs);
+  int* St = reinterpret_cast<int*>(base + l.s_off);
   const int64_t tm = l.nip / kTM, tn = l.njp / kTN;
-  ozaki_gemm_kernel<S><<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, As, Bs, ea, fb, alpha, ni, nj, l.nip, l.njp,
-                                                                    l.nkp, int(tn));
+  ozaki_gemm_kernel<S><<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(St, As, Bs, l.nip, l.njp, l.nkp, int(tn));
+  ozaki_combine_kernel<S><<<dim3(unsigned(ceil_div(ni * nj, 256))), dim3(256), 0, s>>>(C, St, ea, fb, alpha, ni, nj,
+                                                                                       l.nip, l.njp);
   return launch_status();
 }
 
@@ -389,7 +400,8 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
   const int s = nslices == 0 ? 7 : nslices;
   const MixedLayout l = mixed_layout(ni, nj, nk, s);
   if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
-  if (l.nip > INT32_MAX || (l.nip / kTM) * (l.njp / kTN) > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  if (l.nip > INT32_MAX || (l.nip / kTM) * (l.njp / kTN) > INT32_MAX || ceil_div(ni * nj, 256) > INT32_MAX)
+    return RPX_ERR_UNSUPPORTED;
   char* base = reinterpret_cast<char*>(scratch);
   hipStream_t st = as_stream(stream);
   switch (s) {
