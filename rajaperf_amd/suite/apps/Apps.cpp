@@ -72,9 +72,23 @@ void MASS3DPA::tearDown(VariantID, size_t) {
 void MASS3DPA::setMI355XTuningDefinitions(VariantID vid) {
   addVariantTuningName(vid, "default");
   addVariantTuningName(vid, "naive");  // one element per Q1D x Q1D workgroup, the plain FP64 comparator
+  // int8 MFMA on 7 Ozaki slices of the Kronecker form, where the order has it
+  if (rpx_mass3dpa_mixed_scratch_bytes(m_D1D, m_Q1D, 7) > 0) addVariantTuningName(vid, "mixed");
 }
 void MASS3DPA::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
+  if (tune_idx == 2) {  // the sliced bases' scratch is allocated outside the timed reps
+    const size_t bytes = rpx_mass3dpa_mixed_scratch_bytes(m_D1D, m_Q1D, 7);
+    char* scratch = nullptr;
+    allocData(scratch, Size_type(bytes));
+    startTimer();
+    for (RepIndex_type irep = 0; irep < reps; ++irep)
+      rpxCheck(rpx_mass3dpa_mixed_f64(m_B, m_Bt, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, 7, scratch, bytes, nullptr),
+               "MASS3DPA mixed");
+    stopTimer();
+    deallocData(scratch);
+    return;
+  }
   rpx_tuning tune = defaultTuning();
   if (tune_idx == 1) tune.flags |= RPX_TUNE_NAIVE;
   startTimer();

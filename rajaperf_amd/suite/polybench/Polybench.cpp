@@ -37,6 +37,7 @@ POLYBENCH_GEMM::POLYBENCH_GEMM(const RunParams& params) : KernelBase(Polybench_G
 void POLYBENCH_GEMM::setMI355XTuningDefinitions(VariantID vid) {
   addVariantTuningName(vid, "default");  // FP64 on the MFMA f64 units
   addVariantTuningName(vid, "naive");    // one thread per C element, the plain FP64 comparator
+  addVariantTuningName(vid, "mixed");    // int8 MFMA on 7 Ozaki slices, FP64 accuracy (error <= 1.8e-14 |A||B|)
 }
 void POLYBENCH_GEMM::setUp(VariantID, size_t) {
   allocAndInitData(m_A, Size_type(m_ni * m_nk), -1.0, 1.0);
@@ -53,6 +54,18 @@ void POLYBENCH_GEMM::tearDown(VariantID, size_t) {
 }
 void POLYBENCH_GEMM::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
+  if (tune_idx == 2) {  // the slices' scratch is allocated outside the timed reps
+    const size_t bytes = rpx_gemm_mixed_scratch_bytes(m_ni, m_nj, m_nk, 7);
+    char* scratch = nullptr;
+    allocData(scratch, Size_type(bytes));
+    startTimer();
+    for (RepIndex_type irep = 0; irep < reps; ++irep)
+      rpxCheck(rpx_gemm_mixed_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, 7, scratch, bytes, nullptr),
+               "GEMM mixed");
+    stopTimer();
+    deallocData(scratch);
+    return;
+  }
   rpx_tuning tune = defaultTuning();
   if (tune_idx == 1) tune.flags |= RPX_TUNE_NAIVE;
   startTimer();
@@ -82,7 +95,7 @@ POLYBENCH_JACOBI_2D::POLYBENCH_JACOBI_2D(const RunParams& params) : KernelBase(P
   setVariantDefined(MI355X_Native);
 }
 void POLYBENCH_JACOBI_2D::setMI355XTuningDefinitions(VariantID vid) {
-  addVariantTuningName(vid, "default");    // three time steps per fused launch, 8-row tiles
+  addVariantTuningName(vid, "default");    // three time steps per fused launch, 16-row tiles
   addVariantTuningName(vid, "depth_2");    // two time steps per fused launch
   addVariantTuningName(vid, "depth_1");    // one time step per fused launch
   addVariantTuningName(vid, "two_sweep");  // one launch per sweep, as the suite's kernels

@@ -161,3 +161,24 @@ def test_harness_roofline_csv_joins_counters(rpx, tmp_path):
         assert float(r["Checksum diff"]) == 0.0  # every TRIAD tuning is bit-exact
     for r in copy_rows:
         assert r["FETCH_SIZE per dispatch"] == "" and r["WRITE_SIZE per dispatch"] == ""
+
+
+@pytest.mark.parametrize("kernel,size,extra", [("Polybench_GEMM", 100 * 100, []), ("Polybench_GEMM", 97 * 97, ["--nk", "131"]),
+                                               ("Apps_MASS3DPA", 125 * 300, []),
+                                               ("Apps_MASS3DPA", 64 * 200, ["--d1d", "3", "--q1d", "4"])])
+@pytest.mark.parametrize("mode", ["reference", "seeded"])
+def test_harness_mixed_tunings_match_base_seq(rpx, orc, kernel, size, extra, mode):
+    """The "mixed" tunings (int8 MFMA on 7 Ozaki slices) run through
+    KernelBase like the others and keep Base_Seq's checksum to 1e-12."""
+    from fractions import Fraction
+    args = ["-k", kernel, "--size", str(size), "--checkrun", "2", "--input-mode", mode, "-t", "mixed", *extra]
+    res = run_suite(args)
+    assert res["rc"] == 0, res.get("failures")
+    (run,) = [r for r in res["runs"] if r["kernel"] == kernel]
+    assert run["tuning"] == "mixed"
+    kw = dict(size=size, checkrun=2, input_mode=1 if mode == "seeded" else 0)
+    for a, b in zip(extra[::2], extra[1::2]):
+        kw[a.lstrip("-")] = int(b)
+    ref = orc.run(kernel, "Base_Seq", **kw)
+    ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
+    assert abs(float(run["checksum"] - ref_cs)) <= 1e-12 * max(1.0, abs(float(ref_cs)))
