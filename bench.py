@@ -167,6 +167,48 @@ def gpu_run(steps: int, warmup: int):
     return arch, total_ms, per_kernel_ms, skip_b_ms, wall
 
 
+def mixed_vs_fp64(reps: int = 5) -> dict:
+    """The metric's second half: GEMM 4096^3 (alpha 1.5) on int8 MFMA with 7
+    Ozaki slices against this build's FP64 MFMA kernel and the plain FP64
+    comparator (the suite's Base_HIP shape), with the mixed result's largest
+    deviation from the FP64 MFMA result relative to |alpha|(|A||B|)_ij on
+    8 sampled rows.  Not part of `value`."""
+    from rajaperf_amd import abi
+    L = abi.lib()
+    B = abi.DeviceBuffer
+    n, alpha = 4096, 1.5
+    dA, dB, Cf, Cm = B(8 * n * n), B(8 * n * n), B(8 * n * n), B(8 * n * n)
+    L.rpx_fill_uniform_f64(dA.ptr, n * n, SEED, 0, -1.0, 1.0, None)
+    L.rpx_fill_uniform_f64(dB.ptr, n * n, SEED, 1, -1.0, 1.0, None)
+    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 7)
+    scr = B(sb)
+    naive = abi.tuning(flags=abi.RPX_TUNE_NAIVE)
+
+    def timed(fn, k):
+        fn()
+        abi.synchronize()
+        e0, e1 = abi.Event(), abi.Event()
+        e0.record(None)
+        for _ in range(k):
+            fn()
+        e1.record(None)
+        abi.synchronize()
+        return e0.elapsed_ms(e1) / k
+    ms_f64 = timed(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None), reps)
+    ms_mix = timed(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 7, scr.ptr, sb, None),
+                   reps)
+    ms_plain = timed(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, naive, None), 1)
+    rows = np.sort(np.random.default_rng(0).choice(n, 8, replace=False))
+    A = dA.to_array().reshape(n, n)[rows]
+    Bh = dB.to_array().reshape(n, n)
+    scale = alpha * (np.abs(A) @ np.abs(Bh))
+    err = float(np.max(np.abs(Cm.to_array().reshape(n, n)[rows] - Cf.to_array().reshape(n, n)[rows]) / scale))
+    return {"kernel": "Polybench_GEMM 4096^3, alpha 1.5, U[-1,1)", "slices": 7, "fp64_mfma_ms": round(ms_f64, 4),
+            "plain_fp64_ms": round(ms_plain, 3), "mixed_ms": round(ms_mix, 4),
+            "speedup_vs_fp64_mfma": round(ms_f64 / ms_mix, 3), "speedup_vs_plain_fp64": round(ms_plain / ms_mix, 2),
+            "max_err_vs_fp64_rel_abs_ab": err, "stated_error_bound": (7 + 3) * 2.0 ** -49}
+
+
 def cpu_baseline():
     """Oracle Base_OpenMP on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -311,6 +353,7 @@ def main():
                      "avg_launch_ms": round(launch_ms, 5)},
         "kernels": kernels,
         "wall_s": round(wall, 3),
+        "mixed": mixed_vs_fp64(),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
