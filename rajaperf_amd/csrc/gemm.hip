@@ -51,10 +51,6 @@ constexpr int kTM = 128, kTN = 128, kTK = 16;
 constexpr int kThreads = 256;
 constexpr int kLdsRow = kTK + 2;  // doubles per LDS row of either operand
 
-__device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned G) {
-  return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-}
-
 __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restrict__ C,
                                                                  const double* __restrict__ A,
                                                                  const double* __restrict__ B, double alpha,

@@ -77,4 +77,32 @@ __host__ __device__ __forceinline__ double seeded_unit(uint64_t state0, uint64_t
   return static_cast<double>(mix64(state0 + (i + 1) * kGamma) >> 11) * 0x1.0p-53;
 }
 
+// XCD-major numbering: XCD x (workgroups b with b % 8 == x) gets the
+// contiguous range of logical ids [x*G/8, (x+1)*G/8).
+__device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned G) {
+  return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+}
+
+// Output tile (tm, tn) of hardware workgroup b for a GEMM-shaped grid.  With
+// one workgroup per CU, the grid runs in rounds of 256 workgroups, 32 per XCD
+// (workgroup b runs on XCD b % 8).  When both tile counts are multiples of 16, each round covers
+// a 16 x 16 patch of tiles and each XCD a 4 x 8 part of it, so an XCD's L2
+// holds the 4 A tiles and 8 B tiles its 32 workgroups read at every K step
+// (not 1 A and 32 B tiles, as a row-major patch would).  Otherwise tiles
+// are numbered XCD-major.
+__device__ __forceinline__ void tile_of(unsigned b, unsigned G, int tiles_m, int tiles_n, int64_t& tm, int64_t& tn) {
+  if (tiles_m % 16 == 0 && tiles_n % 16 == 0) {
+    const unsigned x = b % 8, slot = b / 8;  // XCD, order within the XCD
+    const unsigned round = slot / 32, pos = slot % 32;
+    const unsigned pn = unsigned(tiles_n) / 16;  // patches per patch row
+    const unsigned patch_m = round / pn, patch_n = round % pn;
+    tm = int64_t(patch_m) * 16 + (x / 2) * 4 + pos / 8;
+    tn = int64_t(patch_n) * 16 + (x % 2) * 8 + pos % 8;
+    return;
+  }
+  const unsigned t = xcd_major(b, G);
+  tm = t / unsigned(tiles_n);
+  tn = t % unsigned(tiles_n);
+}
+
 }  // namespace rpx
