@@ -299,10 +299,17 @@ int rpx_mass3dpa_f64(const double* B, const double* Bt, const double* D,
  * rpx_mass3dpa_batched_plan uploads the batch and chunk tables into
  * `scratch` and waits for that copy (a host synchronisation of `stream`);
  * rpx_mass3dpa_batched_run then runs a planned scratch any number of times
- * (no host synchronisation; it may be captured in a graph): one persistent
- * launch per order (eight launches; an order without batches exits at once),
- * or, with RPX_TUNE_MASS_SINGLE_LAUNCH, one persistent launch for all
- * orders.  rpx_mass3dpa_batched_f64 does both.  tune->grid_size sets the
+ * (no host synchronisation; it may be captured in a graph): one launch per
+ * order present, with one workgroup per chunk of that order, then one short
+ * persistent launch for any chunks those grids missed.  The grids come from
+ * chunk counts that the plan call records on the host, keyed by the scratch
+ * address (the 64 most recent plans).  The closing launch keeps results
+ * right when a record no longer matches the scratch, e.g. a larger plan
+ * copied into a reused address.  Without a record (a plan copied from
+ * another buffer), or with tune->grid_size, the run is one persistent launch
+ * per order (eight launches; an order without batches exits at once).  With
+ * RPX_TUNE_MASS_SINGLE_LAUNCH it is one persistent launch for all orders.
+ * rpx_mass3dpa_batched_f64 does both calls.  tune->grid_size sets the
  * number of persistent workgroups per launch (default: as many as are
  * resident at once; 3 per CU for the single launch). */
 typedef struct rpx_mass_batch {

@@ -29,6 +29,7 @@
 #include "rpx_common.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 using namespace rpx;
@@ -212,12 +213,18 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
 #pragma unroll
     for (int qz = 0; qz < Q; ++qz) v[qz] = in[qz * D2];
     gdouble* yp = Yg + (e0 + e) * D3 + r;
+    // All of the pencil's Y values are loaded before the first store: where
+    // the Y pointer comes from memory the compiler otherwise keeps each
+    // load behind the previous store and waits on it D times per pencil.
+    double y[D];
+#pragma unroll
+    for (int dz = 0; dz < D; ++dz) y[dz] = yp[dz * D2];
 #pragma unroll
     for (int dz = 0; dz < D; ++dz) {
       double u = 0.0;
 #pragma unroll
       for (int qz = 0; qz < Q; ++qz) u += v[qz] * sBt[dz * Q + qz];
-      yp[dz * D2] += u;
+      yp[dz * D2] = y[dz] + u;
     }
   }
 }
@@ -272,6 +279,20 @@ __device__ __attribute__((noinline)) void mass_batch_chunk(rpx_mass_batch b, int
                               bufA, bufB, sB, sB + (D + 1) * D);
 }
 
+// Elements per chunk of a batch of order d1d (Q1D = d1d + 1).
+__host__ __device__ inline int64_t batch_chunk_elems(int d1d) {
+  switch (d1d) {
+    case 1: return MassShape<1, 2>::E;
+    case 2: return MassShape<2, 3>::E;
+    case 3: return MassShape<3, 4>::E;
+    case 4: return MassShape<4, 5>::E;
+    case 5: return MassShape<5, 6>::E;
+    case 6: return MassShape<6, 7>::E;
+    case 7: return MassShape<7, 8>::E;
+    default: return MassShape<8, 9>::E;
+  }
+}
+
 // Scratch of a batched plan: a 256-byte header (the chunk count), the batch
 // descriptors, each chunk's batch, each batch's first chunk.
 struct BatchHeader {
@@ -290,17 +311,32 @@ __host__ __device__ inline size_t chunk0_offset(int64_t nbatch, int64_t nchunks)
   return chunk_batch_offset(nbatch) + align256(size_t(nchunks) * sizeof(int));
 }
 
-__global__ __launch_bounds__(kBS, 3) void mass3dpa_batched_kernel(const char* __restrict__ plan) {
+// Chunks already run, per order: the first done[d] chunks of order d.
+struct ChunkSkip {
+  int64_t done[9];
+};
+
+// Every order's chunks after its first skip.done[d] (all of them for a zero
+// skip), highest order first; plan order puts the orders' ranges in that
+// order, so c only grows.
+__global__ __launch_bounds__(kBS, 3) void mass3dpa_batched_kernel(const char* __restrict__ plan, ChunkSkip skip) {
   __shared__ double lds[kBatchLds];
   const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(plan);
   const int64_t nchunks = hdr->nchunks, nbatch = hdr->nbatch;
   const auto* batches = reinterpret_cast<const rpx_mass_batch*>(plan + 256);
   const auto* chunk_batch = reinterpret_cast<const int*>(plan + chunk_batch_offset(nbatch));
   const auto* chunk0 = reinterpret_cast<const int64_t*>(plan + chunk0_offset(nbatch, nchunks));
-  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const int bi = chunk_batch[c];
-    const rpx_mass_batch b = batches[bi];
-    const int64_t k = c - chunk0[bi];
+  rpx_mass_batch b{};
+  int64_t cb = 0, ce = 0;  // chunk range of the batch held in b
+  for (int d = 8; d >= 1; --d)
+  for (int64_t c = hdr->order_begin[d] + skip.done[d] + blockIdx.x; c < hdr->order_end[d]; c += gridDim.x) {
+    if (c >= ce) {  // the stride left b's chunks (each batch's are contiguous)
+      const int bi = chunk_batch[c];
+      b = batches[bi];
+      cb = chunk0[bi];
+      ce = cb + ceil_div(b.ne, batch_chunk_elems(b.d1d));
+    }
+    const int64_t k = c - cb;
     lds_double* l = (lds_double*)lds;
     switch (b.d1d) {
       case 1: mass_batch_chunk<1>(b, k * MassShape<1, 2>::E, l); break;
@@ -333,15 +369,101 @@ __global__ __launch_bounds__(kBS) void mass3dpa_order_kernel(const char* __restr
   const auto* batches = reinterpret_cast<const rpx_mass_batch*>(plan + 256);
   const auto* chunk_batch = reinterpret_cast<const int*>(plan + chunk_batch_offset(nbatch));
   const auto* chunk0 = reinterpret_cast<const int64_t*>(plan + chunk0_offset(nbatch, nchunks));
+  // Each batch's chunks are contiguous and c only grows, so the batch
+  // descriptor is looked up again only when the stride leaves its range: a
+  // lookup is a chain of three dependent loads, about 1-2 us, against a few
+  // microseconds of work per chunk at the high orders.
+  rpx_mass_batch b{};
+  int64_t cb = 0, ce = 0;  // chunk range of the batch held in b
   for (int64_t c = begin + blockIdx.x; c < end; c += gridDim.x) {
-    const int bi = chunk_batch[c];
-    const rpx_mass_batch b = batches[bi];
-    const int64_t e0 = (c - chunk0[bi]) * S::E;
+    if (c >= ce) {
+      const int bi = chunk_batch[c];
+      b = batches[bi];
+      cb = chunk0[bi];
+      ce = cb + ceil_div(b.ne, int64_t(S::E));
+    }
+    const int64_t e0 = (c - cb) * S::E;
     const int ne = int((b.ne - e0) < S::E ? (b.ne - e0) : S::E);
     mass3dpa_elements<D, D + 1>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0,
                                 ne, bufA, bufB, sB, sBt);
     __syncthreads();  // the next chunk rewrites the LDS
   }
+}
+
+// The batches of one order with one workgroup per chunk, for a grid of the
+// order's chunk count.  Without the persistent loop around it the element
+// code compiles as in the single-order kernel (164 against 118 VGPRs at
+// D1D = 8, where the looped form waits on each LDS basis read of its last
+// stages and runs 12-15% slower at D1D = 7, 8).
+template <int D>
+__global__ __launch_bounds__(kBS) void mass3dpa_order_chunk_kernel(const char* __restrict__ plan) {
+  using S = MassShape<D, D + 1>;
+  __shared__ double bufA[S::E * S::kA];
+  __shared__ double bufB[S::E * S::kB];
+  __shared__ double sB[(D + 1) * D];
+  __shared__ double sBt[D * (D + 1)];
+  const BatchHeader* hdr = reinterpret_cast<const BatchHeader*>(plan);
+  const int64_t nchunks = hdr->nchunks, nbatch = hdr->nbatch;
+  const int64_t c = hdr->order_begin[D] + blockIdx.x;
+  if (c >= hdr->order_end[D]) return;
+  const auto* batches = reinterpret_cast<const rpx_mass_batch*>(plan + 256);
+  const auto* chunk_batch = reinterpret_cast<const int*>(plan + chunk_batch_offset(nbatch));
+  const auto* chunk0 = reinterpret_cast<const int64_t*>(plan + chunk0_offset(nbatch, nchunks));
+  const int bi = chunk_batch[c];
+  const rpx_mass_batch b = batches[bi];
+  const int64_t e0 = (c - chunk0[bi]) * S::E;
+  const int ne = int((b.ne - e0) < S::E ? (b.ne - e0) : S::E);
+  mass3dpa_elements<D, D + 1>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0,
+                              ne, bufA, bufB, sB, sBt);
+}
+
+template <int D>
+int launch_order_chunks(const char* plan, int64_t chunks, hipStream_t s) {
+  if (chunks <= 0) return RPX_SUCCESS;
+  if (chunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  mass3dpa_order_chunk_kernel<D><<<dim3(unsigned(chunks)), dim3(kBS), 0, s>>>(plan);
+  return launch_status();
+}
+
+// Chunk counts per order of the plans written by rpx_mass3dpa_batched_plan,
+// kept on the host by scratch address so that a run sizes its grids without
+// reading the device.  At most kMaxPlanRecords are kept (oldest dropped); a
+// scratch without a record runs on the persistent kernels.
+struct PlanRecord {
+  const void* scratch;
+  int64_t chunks[9];
+};
+constexpr size_t kMaxPlanRecords = 64;
+std::mutex g_plan_mu;
+std::vector<PlanRecord> g_plans;
+
+void forget_plan(const void* scratch) {
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  g_plans.erase(std::remove_if(g_plans.begin(), g_plans.end(),
+                               [&](const PlanRecord& r) { return r.scratch == scratch; }),
+                g_plans.end());
+}
+
+void record_plan(const void* scratch, const int64_t* chunks) {
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  g_plans.erase(std::remove_if(g_plans.begin(), g_plans.end(),
+                               [&](const PlanRecord& r) { return r.scratch == scratch; }),
+                g_plans.end());
+  if (g_plans.size() >= kMaxPlanRecords) g_plans.erase(g_plans.begin());
+  PlanRecord r;
+  r.scratch = scratch;
+  std::copy(chunks, chunks + 9, r.chunks);
+  g_plans.push_back(r);
+}
+
+bool find_plan(const void* scratch, int64_t* chunks) {
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  for (const PlanRecord& r : g_plans)
+    if (r.scratch == scratch) {
+      std::copy(r.chunks, r.chunks + 9, chunks);
+      return true;
+    }
+  return false;
 }
 
 // Workgroups of mass3dpa_order_kernel<D> resident per CU (queried once).
@@ -363,20 +485,6 @@ int launch_order(const char* plan, int cus, int64_t grid_override, hipStream_t s
   if (grid > INT32_MAX) return RPX_ERR_INVALID_ARGUMENT;
   mass3dpa_order_kernel<D><<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(plan);
   return launch_status();
-}
-
-// Elements per chunk of a batch of order d1d (Q1D = d1d + 1).
-int64_t batch_chunk_elems(int d1d) {
-  switch (d1d) {
-    case 1: return MassShape<1, 2>::E;
-    case 2: return MassShape<2, 3>::E;
-    case 3: return MassShape<3, 4>::E;
-    case 4: return MassShape<4, 5>::E;
-    case 5: return MassShape<5, 6>::E;
-    case 6: return MassShape<6, 7>::E;
-    case 7: return MassShape<7, 8>::E;
-    default: return MassShape<8, 9>::E;
-  }
 }
 
 struct BatchLayout {
@@ -798,6 +906,7 @@ int rpx_mass3dpa_batched_plan(const rpx_mass_batch* batches_host, int nbatch, vo
   const BatchLayout l = batch_layout(batches_host, nbatch);
   if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
   if (l.nchunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  forget_plan(scratch);  // recorded again once the new tables are uploaded
   // Chunks of the highest orders (the costliest) come first, so that the
   // persistent workgroups end on cheap chunks; a batch's chunks stay
   // contiguous.
@@ -837,6 +946,9 @@ int rpx_mass3dpa_batched_plan(const rpx_mass_batch* batches_host, int nbatch, vo
   RPX_TRY(hipMemcpyAsync(base + l.chunk0_off, chunk0.data(), chunk0.size() * sizeof(int64_t),
                          hipMemcpyHostToDevice, st));
   RPX_TRY(hipStreamSynchronize(st));  // the host tables must outlive the copies
+  int64_t chunks[9];
+  for (int d = 0; d < 9; ++d) chunks[d] = h.order_end[d] - h.order_begin[d];
+  record_plan(scratch, chunks);
   return RPX_SUCCESS;
 }
 
@@ -849,8 +961,28 @@ int rpx_mass3dpa_batched_run(const void* scratch, const rpx_tuning* tune, void* 
   hipStream_t s = as_stream(stream);
   const int64_t grid_override = tune_or(tune, &rpx_tuning::grid_size, 0);
   if ((tune_flags(tune) & RPX_TUNE_MASS_SINGLE_LAUNCH) == 0) {
-    // One persistent launch per order, highest first (the plan's order).
     int rc = RPX_SUCCESS;
+    int64_t chunks[9];
+    if (grid_override == 0 && find_plan(scratch, chunks)) {
+      // One launch per order present, one workgroup per chunk, highest first.
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<8>(plan, chunks[8], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<7>(plan, chunks[7], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<6>(plan, chunks[6], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<5>(plan, chunks[5], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<4>(plan, chunks[4], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<3>(plan, chunks[3], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<2>(plan, chunks[2], s);
+      if (rc == RPX_SUCCESS) rc = launch_order_chunks<1>(plan, chunks[1], s);
+      if (rc != RPX_SUCCESS) return rc;
+      // The chunks beyond those grids: none when the record matches the
+      // plan in scratch; the rest of a larger plan written there since by
+      // other means (a copy into a reused address).
+      ChunkSkip skip;
+      std::copy(chunks, chunks + 9, skip.done);
+      mass3dpa_batched_kernel<<<dim3(unsigned(cus)), dim3(kBS), 0, s>>>(plan, skip);
+      return launch_status();
+    }
+    // One persistent launch per order, highest first (the plan's order).
     if (rc == RPX_SUCCESS) rc = launch_order<8>(plan, cus, grid_override, s);
     if (rc == RPX_SUCCESS) rc = launch_order<7>(plan, cus, grid_override, s);
     if (rc == RPX_SUCCESS) rc = launch_order<6>(plan, cus, grid_override, s);
@@ -866,7 +998,7 @@ int rpx_mass3dpa_batched_run(const void* scratch, const rpx_tuning* tune, void* 
   // once.  The kernel finds the tables from the counts in the plan's header.
   const int64_t grid = grid_override > 0 ? grid_override : 3 * int64_t(cus);
   if (grid > INT32_MAX) return RPX_ERR_INVALID_ARGUMENT;
-  mass3dpa_batched_kernel<<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(plan);
+  mass3dpa_batched_kernel<<<dim3(unsigned(grid)), dim3(kBS), 0, s>>>(plan, ChunkSkip{});
   return launch_status();
 }
 

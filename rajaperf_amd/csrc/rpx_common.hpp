@@ -38,7 +38,7 @@ inline T tune_or(const rpx_tuning* t, T rpx_tuning::*field, T dflt) {
 }
 inline int tune_flags(const rpx_tuning* t) { return t ? t->flags : 0; }
 
-inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- device helpers -------------------------------------------------------
 

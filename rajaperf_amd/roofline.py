@@ -17,7 +17,8 @@ Configurations (BASELINE workloads):
   4  GEMM 4096^3 (FP64 MFMA, the plain FP64 comparator, mixed precision);
      MASS3DPA of config 3 with its contractions on int8 MFMA
   5  SORT and SCAN over 256 segments, lengths log-uniform in [2^10, 2^20];
-     MASS3DPA over 64 ragged batches of orders 2..8 in one persistent launch;
+     MASS3DPA over 64 ragged batches of orders 2..8 from one plan (one launch
+     per order, and one persistent launch for all orders);
      one pass of all kernels at random sizes, eager and as one HIP graph
 """
 
@@ -349,7 +350,7 @@ def mass_batches(rows, reps, nbatch=64):
     ms_each = _time(per_batch, reps)
     one = abi.tuning(flags=abi.RPX_TUNE_MASS_SINGLE_LAUNCH)
     ms_one = _time(lambda: L.rpx_mass3dpa_batched_run(scr.ptr, one, None), reps)
-    _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one persistent launch per order)", 5, ms, nbytes,
+    _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one launch per order, one workgroup per chunk)", 5, ms, nbytes,
          elements=int(nes.sum()), per_batch_launches_ms=round(ms_each, 5), speedup=round(ms_each / ms, 3))
     _row(rows, "Apps_MASS3DPA(64 ragged batches, D1D 2..8, one persistent launch for all orders)", 5, ms_one,
          nbytes, elements=int(nes.sum()), per_batch_launches_ms=round(ms_each, 5), speedup=round(ms_each / ms_one, 3))

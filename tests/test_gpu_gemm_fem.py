@@ -127,9 +127,10 @@ def _batches(rpx, rng, specs):
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("grid", [0, 1, 7])
 def test_mass3dpa_ragged_batches_one_launch(rpx, orc, rng, grid, single):
-    """Ragged batches of mixed orders -- one persistent launch per order, or
-    one for all orders -- equal one rpx_mass3dpa_f64 call per batch, bit for
-    bit, and the oracle.  Two batches share order 8, one batch is empty."""
+    """Ragged batches of mixed orders -- one launch per order (one workgroup
+    per chunk, or persistent with a grid size), or one persistent launch for
+    all orders -- equal one rpx_mass3dpa_f64 call per batch, bit for bit, and
+    the oracle.  Two batches share order 8, one batch is empty."""
     specs = [(300, 2), (0, 5), (1, 8), (2049, 4), (77, 7), (513, 3), (40, 6), (9, 1), (130, 8)]
     data = _batches(rpx, rng, specs)
     arr = (rpx.MassBatch * len(data))()
@@ -159,6 +160,51 @@ def test_mass3dpa_batches_reject_bad_orders(rpx, rng):
     scr = rpx.DeviceBuffer(sb)
     with pytest.raises(rpx.RpxError):
         L.rpx_mass3dpa_batched_f64(arr, 1, scr.ptr, sb, None, None)
+
+
+def test_mass3dpa_batched_plan_copied_runs_persistent(rpx, rng):
+    """A plan copied into another buffer has no host record of its chunk
+    counts: the run falls back to the persistent launches, bit-identical."""
+    specs = [(700, 3), (95, 8), (1, 6), (260, 5)]
+    data = _batches(rpx, rng, specs)
+    arr = (rpx.MassBatch * len(data))()
+    for i, (ne, d, host, bufs) in enumerate(data):
+        arr[i] = rpx.MassBatch(*[b.ptr for b in bufs], ne, d, d + 1)
+    L = rpx.lib()
+    sb = L.rpx_mass3dpa_batched_scratch_bytes(arr, len(data))
+    scr, copy = rpx.DeviceBuffer(sb), rpx.DeviceBuffer(sb)
+    L.rpx_mass3dpa_batched_plan(arr, len(data), scr.ptr, sb, None)
+    L.rpx_memcpy_d2d(copy.ptr, scr.ptr, sb, None)
+    L.rpx_mass3dpa_batched_run(copy.ptr, None, None)
+    for ne, d, host, bufs in data:
+        ref_bufs = [dev(rpx, a) for a in host]
+        L.rpx_mass3dpa_f64(*[b.ptr for b in ref_bufs], ne, d, d + 1, None, None)
+        assert np.array_equal(bufs[4].to_array(), ref_bufs[4].to_array())
+
+
+def test_mass3dpa_batched_stale_record_runs_every_chunk(rpx, rng):
+    """A small plan recorded for a scratch, then a larger plan copied over it:
+    the run's per-chunk grids come from the stale record, and its closing
+    persistent launch runs the chunks they miss, so results stay exact."""
+    small = _batches(rpx, rng, [(30, 8), (64, 2)])
+    big = _batches(rpx, rng, [(900, 8), (33, 7), (2000, 2), (410, 5)])
+    L = rpx.lib()
+    arrs = []
+    for data in (small, big):
+        arr = (rpx.MassBatch * len(data))()
+        for i, (ne, d, host, bufs) in enumerate(data):
+            arr[i] = rpx.MassBatch(*[b.ptr for b in bufs], ne, d, d + 1)
+        arrs.append(arr)
+    sb = max(L.rpx_mass3dpa_batched_scratch_bytes(a, len(a)) for a in arrs)
+    scr, other = rpx.DeviceBuffer(sb), rpx.DeviceBuffer(sb)
+    L.rpx_mass3dpa_batched_plan(arrs[0], len(arrs[0]), scr.ptr, sb, None)
+    L.rpx_mass3dpa_batched_plan(arrs[1], len(arrs[1]), other.ptr, sb, None)
+    L.rpx_memcpy_d2d(scr.ptr, other.ptr, sb, None)
+    L.rpx_mass3dpa_batched_run(scr.ptr, None, None)
+    for ne, d, host, bufs in big:
+        ref_bufs = [dev(rpx, a) for a in host]
+        L.rpx_mass3dpa_f64(*[b.ptr for b in ref_bufs], ne, d, d + 1, None, None)
+        assert np.array_equal(bufs[4].to_array(), ref_bufs[4].to_array())
 
 
 def test_mass3dpa_batched_plan_once_run_twice(rpx, rng):
