@@ -39,6 +39,15 @@ namespace {
 constexpr int kBS = 256;
 constexpr int kLdsBudget = 48 * 1024;
 
+// MASS3DPA's LDS bytes per workgroup by order, which sets its elements per
+// workgroup: below 48 KB a CU holds four workgroups where the registers
+// allow it.  Timed with 24, 32, 36, 40 and 48 KB at NE = 262144 (65536 for
+// D1D >= 5): 24 KB is ~10% faster at D1D = 3, 36 KB 2-6% at 4, 40 KB ~6% at
+// 5; D1D = 2 and >= 6 show no gain.
+constexpr int mass_lds_budget(int D) {
+  return D == 3 ? 24 * 1024 : D == 4 ? 36 * 1024 : D == 5 ? 40 * 1024 : kLdsBudget;
+}
+
 // A compiler barrier at the top of each pencil iteration (orders D1D >= 7):
 // the basis values are then re-read from LDS (a wavefront-uniform
 // broadcast) for every pencil instead of all D*Q of them being hoisted into
@@ -61,7 +70,7 @@ struct MassShape {
   static constexpr int kA = Q * Q * Q;  // DDQ / QQQ / QDD per element
   static constexpr int kB = D * Q * Q;  // DQQ / QQD per element
   static constexpr int kBytes = 8 * (kA + kB);
-  static constexpr int E0 = kLdsBudget / kBytes;
+  static constexpr int E0 = mass_lds_budget(D) / kBytes;
   static constexpr int E = E0 < 1 ? 1 : (E0 > 64 ? 64 : E0);
   static constexpr int kLds = E * (kA + kB) + 2 * Q * D;  // doubles of LDS
 };
