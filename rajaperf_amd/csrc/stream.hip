@@ -55,7 +55,7 @@ __global__ __launch_bounds__(BS) void stream_vec_kernel(double* __restrict__ out
   const v2d* __restrict__ yv = reinterpret_cast<const v2d*>(y);
   v2d* __restrict__ ov = reinterpret_cast<v2d*>(out);
 
-  const int64_t base = int64_t(blockIdx.x) * (BS * U) + threadIdx.x;
+  const int64_t base = int64_t(xcd_major(blockIdx.x, gridDim.x)) * (BS * U) + threadIdx.x;
   if (base + (U - 1) * BS < nvec) {
     // Full tile: no bounds checks.
     v2d a[U], b[U];
