@@ -9,12 +9,13 @@
 //  * a tile is 256 work-items x IPT doubles (IPT 16, 32 or 64: 32-128 KiB).
 //    Each wavefront owns a contiguous quarter of the tile and reads it as
 //    IPT/2 rows of 1 KiB (one 16-byte load per lane per row); it scans the
-//    rows in order in registers -- lane pair sum, a 64-lane shuffle scan and
-//    a running row carry -- so no LDS staging buffer limits residency, and
-//    the result leaves through the same 16-byte row stores.  Row loads and
-//    stores are nontemporal unless RPX_TUNE_CACHED is set.  Larger tiles
-//    mean fewer look-back hops: 2^28 doubles scan at 4.0, 4.9 and 5.5 TB/s
-//    with IPT 16, 32 and 64 on MI355X;
+//    rows in order in registers -- lane pair sum, a 64-lane scan on DPP row
+//    shifts and row broadcasts, and a running row carry -- so no LDS buffer
+//    limits residency, and the result leaves through the same 16-byte row
+//    stores.  Row loads and stores are nontemporal unless RPX_TUNE_CACHED is
+//    set.  Larger tiles mean fewer look-back hops: 2^28 doubles scan at 4.2,
+//    5.1 and 6.0 TB/s with IPT 16, 32 and 64 on MI355X (4.0, 4.9 and 5.5
+//    TB/s with the 64-lane scan on ds_bpermute shuffles);
 //  * the tile id is the workgroup id: each XCD dispatches its workgroups in
 //    id order, so the lowest unfinished tile is always resident and the
 //    look-back always makes progress.  (A ticket counter taken with one
@@ -74,6 +75,43 @@ __device__ __forceinline__ void store_value(double* p, double v) {
   __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
                      static_cast<unsigned long long>(__double_as_longlong(v)), __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wavefront scans on DPP lane moves (VALU only, no LDS round trips).  A
+// 64-bit value moves as two 32-bit halves; lanes whose source is outside
+// the pattern, or in a row the mask leaves out, receive 0.
+template <int CTRL, int ROW_MASK, bool ZERO_FILL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROW_MASK, 0xF, ZERO_FILL);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROW_MASK, 0xF, ZERO_FILL);
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(unsigned(hi)) << 32) |
+                                                     unsigned(lo)));
+}
+
+// Inclusive scan over the 64 lanes: shifts by 1, 2, 4 and 8 within each row
+// of 16 lanes, then lane 15's row total into rows 1 and 3 (row_bcast:15)
+// and lane 31's half total into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ double wave_inclusive_scan_dpp(double v) {
+  v += dpp_move<0x111, 0xF, false>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xF, false>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xF, false>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xF, false>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xA, false>(v);  // row_bcast:15
+  v += dpp_move<0x143, 0xC, false>(v);  // row_bcast:31
+  return v;
+}
+
+// Lane l receives lane l-1's value, lane 0 receives 0 (wave_shr:1).
+__device__ __forceinline__ double wave_shift_up_dpp(double v) { return dpp_move<0x138, 0xF, true>(v); }
+
+// Lane 63's value, wavefront-uniform.
+__device__ __forceinline__ double last_lane(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), kWave - 1);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), kWave - 1);
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(unsigned(hi)) << 32) |
+                                                     unsigned(lo)));
 }
 
 // Segment tables of a segmented scan: segment s covers [off[s], off[s+1])
@@ -144,15 +182,14 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
   }
 
   // Exclusive scan inside the wavefront, row by row: lane pair sums, a
-  // 64-lane scan, and a running carry of the row totals.
+  // 64-lane scan on DPP moves, and a running carry of the row totals.
   double carry = 0.0;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     const double p = v[r].x + v[r].y;
-    const double inc = wave_inclusive_scan(p, lane);
-    double exc = __shfl_up(inc, 1, kWave);
-    if (lane == 0) exc = 0.0;
-    const double row_total = __shfl(inc, kWave - 1, kWave);
+    const double inc = wave_inclusive_scan_dpp(p);
+    const double exc = wave_shift_up_dpp(inc);
+    const double row_total = last_lane(inc);
     const double e0 = carry + exc;
     v[r] = v2d{e0, e0 + v[r].x};
     carry += row_total;
