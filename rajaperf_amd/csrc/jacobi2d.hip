@@ -27,17 +27,26 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 
 constexpr int kBS = 256;             // work-items per tile row
 constexpr int kCols = 2 * kBS;       // columns per tile
-constexpr int kXcds = 8;
 
 __device__ __forceinline__ double point(double c, double w, double e, double s, double n) {
   return 0.2 * ((((c + w) + e) + s) + n);
 }
 
-// XCD-aware tile id: hardware block b runs on XCD b % 8; give XCD x the
-// contiguous range of logical tiles [x*G/8, (x+1)*G/8).
-__device__ __forceinline__ unsigned logical_tile(unsigned b, unsigned G) {
-  if (G % kXcds != 0) return b;
-  return (b % kXcds) * (G / kXcds) + b / kXcds;
+// XCD-aware tile id: hardware block b runs on XCD b % 8; XCD x walks a
+// contiguous range of logical tiles.
+__device__ __forceinline__ unsigned logical_tile(unsigned b, unsigned G) { return xcd_major(b, G); }
+
+// lane l receives v from lane l-1; lane 0 receives 0.
+__device__ __forceinline__ double lane_shr_zero(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+// lane l receives v from lane l+1; lane 63 receives 0.
+__device__ __forceinline__ double lane_shl_zero(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
 }
 
 template <int R>
@@ -82,8 +91,8 @@ __global__ __launch_bounds__(kBS) void jacobi_vec_kernel(double* __restrict__ ds
   for (int k = 0; k < R; ++k) {
     const v2d c = v[k + 1];
     // West of j0 is the previous lane's .y; east of j0+1 is the next lane's .x.
-    double w = __shfl_up(c.y, 1, kWave);
-    double e = __shfl_down(c.x, 1, kWave);
+    double w = lane_shr_zero(c.y);
+    double e = lane_shl_zero(c.x);
     if (lane == 0) w = wh[k];
     if (lane == kWave - 1) e = eh[k];
     if (k < rows) {
@@ -139,18 +148,6 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
 // new A); R = 16 rows per wavefront is the fastest tile.
 constexpr int kStripCols = 128;
 
-// lane l receives v from lane l-1; lane 0 receives 0.
-__device__ __forceinline__ double lane_shr_zero(double v) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
-  return __hiloint2double(hi, lo);
-}
-// lane l receives v from lane l+1; lane 63 receives 0.
-__device__ __forceinline__ double lane_shl_zero(double v) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
-  return __hiloint2double(hi, lo);
-}
 
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 constexpr int kAhead = 6;  // level-0 rows loaded ahead of their use
