@@ -127,8 +127,10 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
 //
 // One wavefront owns a strip of 128 columns (its lanes hold column pairs) and
 // a chunk of R output rows.  With H = 2T sweeps per launch the strip starts at
-// j0 = (128-2H)*s - H and writes the 128-2H columns [j0+H, j0+128-H); the H
-// columns at each edge are halo.  The wavefront loads R+2H rows of A (level 0)
+// j0 = 112*s - H and writes the 112 columns [j0+H, j0+H+112), whole 128-byte
+// lines when rows start on one (writing 128-2H columns per strip instead, so
+// that stores start mid-line, made a 50-step run at N = 8192 3.4% slower);
+// the columns on either side are halo.  The wavefront loads R+2H rows of A (level 0)
 // and runs a cascade of levels 1..2T: at step q, level L produces its row
 // rbase+q-L from the last three rows of level L-1, held in a 3-row rolling
 // register window.  West/east neighbours come from the adjacent lanes.  Odd
@@ -147,6 +149,12 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
 // N = 8192, a launch of depth 3 moves the ideal 1.08 GB (read A, write the
 // new A); R = 16 rows per wavefront is the fastest tile.
 constexpr int kStripCols = 128;
+// Output columns per strip, at every depth: 112 columns are 7 whole 128-byte
+// lines, and strip s writes columns [112 s, 112 s + 112), so where rows start
+// on a line (N a multiple of 16) every store covers whole lines and no line
+// is written by two strips.  A strip loads columns [112 s - H, 112 s - H + 128),
+// which leaves at least H halo columns on both sides for H = 2T <= 8.
+constexpr int kOwnCols = 112;
 
 
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
@@ -180,7 +188,8 @@ __device__ __forceinline__ void fused_tile(double* __restrict__ Aout, double* __
   const int voff = lane * 16;
   const int64_t jl = j0 + 2 * lane;       // first column of this lane's pair
   const bool col_in = jl >= 0 && jl < N;  // N even and jl even: jl+1 < N too
-  const bool own = lane >= T && lane < kWave - T;  // columns [j0+H, j0+128-H)
+  static_assert(2 * H <= kStripCols - kOwnCols, "halo columns on both sides of the owned ones");
+  const bool own = lane >= T && lane < T + kOwnCols / 2;  // columns [j0+H, j0+H+112)
   const bool x_out = own && jl >= 1 && jl <= N - 2;
   const bool y_out = own && jl + 1 >= 1 && jl + 1 <= N - 2;
 
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ 
   if (wave_global >= total) return;
   const int strip = int(wave_global % unsigned(strips));  // a workgroup spans 4 adjacent strips
   const int chunk = int(wave_global / unsigned(strips));
-  const int64_t j0 = int64_t(strip) * (kStripCols - 2 * H) - H;
+  const int64_t j0 = int64_t(strip) * kOwnCols - H;
   const int64_t r0 = 1 + int64_t(chunk) * R;
   int64_t rend = r0 + R;
   if (rend > N - 1) rend = N - 1;
@@ -294,8 +303,7 @@ __global__ __launch_bounds__(256) void boundary_copy_kernel(double* __restrict__
 
 template <int R, int T>
 int launch_fused(double* Aout, double* B, const double* Ain, int64_t N, bool write_b, hipStream_t s) {
-  const int64_t valid = kStripCols - 4 * T;
-  const int64_t strips = ceil_div(N - 1, valid);  // output columns [0, N-1)
+  const int64_t strips = ceil_div(N - 1, kOwnCols);  // output columns [0, N-1)
   const int64_t chunks = ceil_div(N - 2, R);
   const int64_t grid = ceil_div(strips * chunks, 4);
   if (grid > INT32_MAX || strips > INT32_MAX || chunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
