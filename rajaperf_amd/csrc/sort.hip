@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBS) void hist_kernel(const u64* __restrict__ src, 
                                                    int shift, int convert, int src_is_x) {
   __shared__ unsigned h[kBS / kWave][kRadix];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const TileInfo ti = get_tile(plan, blockIdx.x);
+  const TileInfo ti = get_tile(plan, xcd_major(blockIdx.x, gridDim.x));
   const long long sbase = (src_is_x ? ti.seg_x : ti.seg_alt) + (long long)ti.local * kTile;
 #pragma unroll
   for (int w = 0; w < kBS / kWave; ++w) h[w][tid] = 0u;
@@ -170,6 +170,11 @@ __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ cou
   }
 }
 
+// Tiles are numbered XCD-major (each XCD takes a contiguous range of a
+// segment's tiles): the partial 128-byte lines at the ends of adjacent tiles'
+// digit runs are then written from one L2, close together in time.  Tile
+// order does not change the result (every position comes from the offsets);
+// measured 2.47 -> 2.33 ms on 256 ragged segments (35.8 M keys).
 __global__ __launch_bounds__(kBS) void scatter_kernel(const u64* __restrict__ src, u64* __restrict__ dst,
                                                       const unsigned* __restrict__ counts, Plan plan, int shift,
                                                       int convert_in, int convert_out, int src_is_x) {
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(kBS) void scatter_kernel(const u64* __restrict__ sr
   __shared__ unsigned wsum[kBS / kWave];
   __shared__ u64 sorted[kTile];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const TileInfo ti = get_tile(plan, blockIdx.x);
+  const TileInfo ti = get_tile(plan, xcd_major(blockIdx.x, gridDim.x));
   const long long sbase = (src_is_x ? ti.seg_x : ti.seg_alt) + (long long)ti.local * kTile;
   const long long dbase = src_is_x ? ti.seg_alt : ti.seg_x;
 #pragma unroll
