@@ -647,10 +647,35 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
     sB[t] = be;
     sG[t] = ge;
   }
+  // Register budget.  Hoisting the 2*Q*D basis values out of the pencil
+  // loops costs more VGPRs than it saves LDS reads here (each basis read
+  // feeds three components), so every pencil iteration re-reads them from
+  // LDS (a wavefront-uniform broadcast); the registers go to the work-item's
+  // D values, loaded before P1 so that their latency overlaps two stages.
+  // Measured at NE = 262144 (65536 for D1D >= 5): (4,5) 0.447 -> 0.408 ms
+  // with uniform D, 0.49 -> 0.47 ms with the symmetric-tensor D of the
+  // roofline report (160 VGPRs, three wavefronts per SIMD; prefetching D
+  // alone took 232); (5,6) 0.404 -> 0.237 ms; (6,7) 0.651 -> 0.438 ms.  The
+  // non-symmetric (4,5) case, which has no prefetch, was 2% slower
+  // re-reading, so it keeps the hoist below D1D = 5.
+  constexpr bool kReload = SYMM || D >= 5;
+  constexpr bool kPreD = SYMM && E * Q2 <= kBS;  // one P3 pencil per work-item
+  double dpre[kPreD ? 6 : 1][Q];
+  if constexpr (kPreD) {
+    if (tid < ne * Q2) {
+      const int e = tid / Q2, r = tid % Q2;
+      const double* dp = Dg + (e0 + e) * (NC * Q3) + r;
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) dpre[c][qz] = dp[c * Q3 + qz * Q2];
+    }
+  }
   __syncthreads();
 
   // P1 (DIFFUSION3DPA_3): DDQ0 = sum X*B, DDQ1 = sum X*G over dx.
   for (int w = tid; w < ne * D2; w += kBS) {
+    if constexpr (kReload) reload_basis();
     const int e = w / D2, r = w % D2, dz = r / D, dy = r % D;
     const double* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
     double x[D];
@@ -672,6 +697,7 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
   __syncthreads();
   // P2 (DIFFUSION3DPA_4): DQQ0 = sum DDQ1*B, DQQ1 = sum DDQ0*G, DQQ2 = sum DDQ0*B over dy.
   for (int w = tid; w < ne * D * Q; w += kBS) {
+    if constexpr (kReload) reload_basis();
     const int e = w / (D * Q), r = w % (D * Q), dz = r / Q, qx = r % Q;
     const double* in = bufA + e * S::kA + dz * D * Q + qx;
     double a0[D], a1[D];
@@ -698,6 +724,7 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
   __syncthreads();
   // P3 (DIFFUSION3DPA_5): gradient at the point, times the symmetric tensor.
   for (int w = tid; w < ne * Q2; w += kBS) {
+    if constexpr (kReload) reload_basis();
     const int e = w / Q2, r = w % Q2;
     const double* in = bufB + e * S::kB + r;
     double c0[D], c1[D], c2[D];
@@ -719,15 +746,16 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
         ww += c2[dz] * sG[qz * D + dz];
       }
       const double* o = dp + qz * Q2;
-      const double O11 = o[0];
-      const double O12 = o[Q3];
-      const double O13 = o[2 * Q3];
+      // Symmetric storage order (11, 12, 13, 22, 23, 33), DIFFUSION3DPA.hpp:384-392.
+      const double O11 = kPreD ? dpre[0][qz] : o[0];
+      const double O12 = kPreD ? dpre[1][qz] : o[Q3];
+      const double O13 = kPreD ? dpre[2][qz] : o[2 * Q3];
       const double O21 = SYMM ? O12 : o[3 * Q3];
-      const double O22 = SYMM ? o[3 * Q3] : o[4 * Q3];
-      const double O23 = SYMM ? o[4 * Q3] : o[5 * Q3];
+      const double O22 = kPreD ? dpre[3][qz] : (SYMM ? o[3 * Q3] : o[4 * Q3]);
+      const double O23 = kPreD ? dpre[4][qz] : (SYMM ? o[4 * Q3] : o[5 * Q3]);
       const double O31 = SYMM ? O13 : o[6 * Q3];
       const double O32 = SYMM ? O23 : o[7 * Q3];
-      const double O33 = SYMM ? o[5 * Q3] : o[8 * Q3];
+      const double O33 = kPreD ? dpre[5][qz] : (SYMM ? o[5 * Q3] : o[8 * Q3]);
       out[qz * Q2] = ((O11 * u) + (O12 * v)) + (O13 * ww);
       out[A1 + qz * Q2] = ((O21 * u) + (O22 * v)) + (O23 * ww);
       out[2 * A1 + qz * Q2] = ((O31 * u) + (O32 * v)) + (O33 * ww);
@@ -736,6 +764,7 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
   __syncthreads();
   // P4 (DIFFUSION3DPA_7): QQD0 = sum QQQ0*Gt, QQD1 = sum QQQ1*Bt, QQD2 = sum QQQ2*Bt over qx.
   for (int w = tid; w < ne * Q2; w += kBS) {
+    if constexpr (kReload) reload_basis();
     const int e = w / Q2, r = w % Q2;  // r = qz*Q + qy
     const double* in = bufA + e * S::kA + r * Q;
     double c0[Q], c1[Q], c2[Q];
@@ -763,6 +792,7 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
   __syncthreads();
   // P5 (DIFFUSION3DPA_8): QDD0 = sum QQD0*Bt, QDD1 = sum QQD1*Gt, QDD2 = sum QQD2*Bt over qy.
   for (int w = tid; w < ne * Q * D; w += kBS) {
+    if constexpr (kReload) reload_basis();
     const int e = w / (Q * D), r = w % (Q * D), qz = r / D, dx = r % D;
     const double* in = bufB + e * S::kB + qz * Q * D + dx;
     double c0[Q], c1[Q], c2[Q];
@@ -790,6 +820,7 @@ __global__ __launch_bounds__(kBS) void diffusion3dpa_kernel(const double* __rest
   __syncthreads();
   // P6 (DIFFUSION3DPA_9): sum over qz, Y += (u + v) + w.
   for (int w = tid; w < ne * D2; w += kBS) {
+    if constexpr (kReload) reload_basis();
     const int e = w / D2, r = w % D2;
     const double* in = bufA + e * S::kA + r;
     double c0[Q], c1[Q], c2[Q];
