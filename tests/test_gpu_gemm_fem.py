@@ -93,8 +93,11 @@ def test_diffusion3dpa_bit_exact(rpx, orc, rng, d1d, kind):
     assert np.array_equal(bufs[4].to_array(), orc.diffusion3dpa(b, g, D, X, Y0, NE, d1d, q1d))
 
 
-def test_diffusion3dpa_full_tensor(rpx, orc, rng):
-    d1d, q1d, NE = 3, 4, 11
+@pytest.mark.parametrize("d1d", [3, 5, 8])
+def test_diffusion3dpa_full_tensor(rpx, orc, rng, d1d):
+    """Non-symmetric 9-component tensor: bit-exact with the oracle below and
+    above D1D = 5, where the kernel switches to re-reading its basis."""
+    q1d, NE = d1d + 1, 11
     b, g = orc.fem_basis(d1d, q1d)
     D = rng.uniform(-1, 1, NE * 9 * q1d ** 3)
     X = rng.uniform(-1, 1, NE * d1d ** 3)
