@@ -97,9 +97,10 @@ int rpx_device_synchronize(void);
  * Executor::runSuite, Executor.cpp:645-670, launch kernel by kernel).
  * Between begin and end, calls on `stream` (a created stream, not the null
  * stream) are recorded instead of run; rpx_graph_launch replays them.  Entry
- * points that synchronise the host (the segmented SCAN and ragged SORT
- * table uploads, rpx_mass3dpa_batched_plan, the scratch status queries)
- * cannot be captured; every other kernel entry point can. */
+ * points that upload host tables (the segmented SCAN, the ragged SORT and
+ * rpx_mass3dpa_batched_plan) or read results back (the scratch status
+ * queries) cannot be captured and return an error under capture; every
+ * other kernel entry point can. */
 int rpx_graph_begin_capture(void* stream);
 int rpx_graph_end_capture(void* stream, void** graph_exec_host);
 int rpx_graph_launch(void* graph_exec, void* stream);
@@ -195,9 +196,10 @@ int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n,
 /* Segmented form: segment s covers [offsets[s], offsets[s+1]) and is scanned
  * independently; `offsets_host` is a host array of nseg+1 ascending values.
  * All segments run in one launch (tiles of 16384 doubles, the look-back of
- * a tile stops at its segment's first tile).  The call uploads the segment
- * tables into `scratch` and waits for that copy (a host synchronisation of
- * `stream`); the scratch needs no initialisation. */
+ * a tile stops at its segment's first tile).  The call copies the segment
+ * tables into `scratch` on `stream` from the library's pinned staging
+ * memory, without a host synchronisation (a staging slot is reused once
+ * its copy has completed); the scratch needs no initialisation. */
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host,
                                         int64_t nseg);
 int rpx_scan_exclusive_segmented_f64(const double* x, double* y,
@@ -212,7 +214,8 @@ int rpx_scan_exclusive_segmented_f64(const double* x, double* y,
  * -inf (sign bit set).  Segment s covers [offsets[s], offsets[s+1]) with
  * `offsets_host` a host array; each segment is sorted independently, in
  * place.  The suite's layout (reps segments of length n) is
- * offsets[s] = s*n. */
+ * offsets[s] = s*n.  Other layouts copy a tile table into `scratch` on
+ * `stream` from pinned staging memory, without a host synchronisation. */
 size_t rpx_sort_segmented_scratch_bytes(const int64_t* offsets_host,
                                         int64_t nseg);
 int rpx_sort_segmented_f64(double* x, const int64_t* offsets_host,

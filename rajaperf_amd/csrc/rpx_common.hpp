@@ -30,6 +30,19 @@ inline int launch_status() { return hip_status(hipGetLastError()); }
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Host tables sent to device memory on a stream without a host
+// synchronisation (runtime.hip): the bytes are copied into library-owned
+// pinned staging memory and sent with hipMemcpyAsync, and a staging slot is
+// reused only once the copies that read it have completed (one event per
+// slot).  Under stream capture the copies would read the slot at every
+// replay, so there it returns RPX_ERR_UNSUPPORTED.
+struct HostChunk {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+int upload_async(const HostChunk* chunks, int n, hipStream_t s);
+
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 template <typename T>

@@ -5,10 +5,79 @@
 // (src/common/HipDataUtils.hpp:141-255) and of DataUtils' initialisers
 // (src/common/DataUtils.cpp:477-595).
 #include <cstring>
+#include <mutex>
 
 #include "rpx_common.hpp"
 
 using namespace rpx;
+
+// ---- asynchronous table uploads ---------------------------------------------
+
+namespace {
+
+struct StageSlot {
+  void* host = nullptr;  // pinned staging memory
+  size_t cap = 0;
+  hipEvent_t done = nullptr;  // recorded after the copies that read `host`
+  int device = -1;            // device of `done`
+  bool pending = false;
+};
+constexpr int kStageSlots = 4;
+std::mutex g_stage_mu;
+StageSlot g_stage[kStageSlots];
+int g_stage_next = 0;
+
+inline size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
+
+}  // namespace
+
+namespace rpx {
+
+int upload_async(const HostChunk* chunks, int n, hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  RPX_TRY(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) return RPX_ERR_UNSUPPORTED;
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += align16(chunks[i].bytes);
+  if (total == 0) return RPX_SUCCESS;
+  int dev = 0;
+  RPX_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_stage_mu);
+  StageSlot& slot = g_stage[g_stage_next];
+  g_stage_next = (g_stage_next + 1) % kStageSlots;
+  if (slot.pending) {  // the slot's previous copies, long done in practice
+    RPX_TRY(hipEventSynchronize(slot.done));
+    slot.pending = false;
+  }
+  if (slot.done && slot.device != dev) {
+    RPX_TRY(hipEventDestroy(slot.done));
+    slot.done = nullptr;
+  }
+  if (!slot.done) {
+    RPX_TRY(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
+    slot.device = dev;
+  }
+  if (slot.cap < total) {
+    if (slot.host) RPX_TRY(hipHostFree(slot.host));
+    slot.host = nullptr;
+    slot.cap = 0;
+    RPX_TRY(hipHostMalloc(&slot.host, total, hipHostMallocDefault));
+    slot.cap = total;
+  }
+  char* p = static_cast<char*>(slot.host);
+  size_t o = 0;
+  for (int i = 0; i < n; ++i) {
+    if (chunks[i].bytes == 0) continue;
+    std::memcpy(p + o, chunks[i].src, chunks[i].bytes);
+    RPX_TRY(hipMemcpyAsync(chunks[i].dst, p + o, chunks[i].bytes, hipMemcpyHostToDevice, s));
+    o += align16(chunks[i].bytes);
+  }
+  RPX_TRY(hipEventRecord(slot.done, s));
+  slot.pending = true;
+  return RPX_SUCCESS;
+}
+
+}  // namespace rpx
 
 namespace {
 

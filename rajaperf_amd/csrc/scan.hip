@@ -415,11 +415,10 @@ int rpx_scan_exclusive_segmented_f64(const double* x, double* y, const int64_t* 
   auto* d_tile_seg = reinterpret_cast<int*>(base + g.tile_seg_off);
   auto* d_off = reinterpret_cast<int64_t*>(base + g.off_off);
   auto* d_tile0 = reinterpret_cast<int64_t*>(base + g.tile0_off);
-  RPX_TRY(hipMemcpyAsync(d_tile_seg, tile_seg.data(), tile_seg.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  RPX_TRY(hipMemcpyAsync(d_off, offsets_host, size_t(nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  RPX_TRY(hipMemcpyAsync(d_tile0, tile0.data(), tile0.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  // The host tables must outlive the copies.
-  RPX_TRY(hipStreamSynchronize(st));
+  const HostChunk tables[3] = {{d_tile_seg, tile_seg.data(), tile_seg.size() * sizeof(int)},
+                               {d_off, offsets_host, size_t(nseg + 1) * sizeof(int64_t)},
+                               {d_tile0, tile0.data(), tile0.size() * sizeof(int64_t)}};
+  if (const int rc = upload_async(tables, 3, st); rc != RPX_SUCCESS) return rc;
   // Flags start at zero for every call (no epochs across segments).
   auto* flags = reinterpret_cast<unsigned long long*>(base + g.l.flags_off);
   const int64_t words = g.ntiles * 2;

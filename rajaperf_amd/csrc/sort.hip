@@ -376,11 +376,9 @@ int rpx_sort_segmented_f64(double* x, const int64_t* offsets_host, int64_t nseg,
     TileInfo* dt = reinterpret_cast<TileInfo*>(base + off);
     off += align256(size_t(s.total_tiles) * sizeof(TileInfo));
     SegInfo* ds = reinterpret_cast<SegInfo*>(base + off);
-    RPX_TRY(hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(TileInfo), hipMemcpyHostToDevice, st));
-    RPX_TRY(hipMemcpyAsync(ds, segs.data(), segs.size() * sizeof(SegInfo), hipMemcpyHostToDevice, st));
-    // The tables are staged from pageable memory owned by this call: wait
-    // for the copies before it goes out of scope.
-    RPX_TRY(hipStreamSynchronize(st));
+    const HostChunk tables[2] = {{dt, tiles.data(), tiles.size() * sizeof(TileInfo)},
+                                 {ds, segs.data(), segs.size() * sizeof(SegInfo)}};
+    if (const int rc = upload_async(tables, 2, st); rc != RPX_SUCCESS) return rc;
     plan.tiles = dt;
     plan.segs = ds;
     nseg_launch = int64_t(segs.size());
