@@ -66,13 +66,20 @@ int upload_async(const HostChunk* chunks, int n, hipStream_t s) {
   }
   char* p = static_cast<char*>(slot.host);
   size_t o = 0;
-  for (int i = 0; i < n; ++i) {
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < n && e == hipSuccess; ++i) {
     if (chunks[i].bytes == 0) continue;
     std::memcpy(p + o, chunks[i].src, chunks[i].bytes);
-    RPX_TRY(hipMemcpyAsync(chunks[i].dst, p + o, chunks[i].bytes, hipMemcpyHostToDevice, s));
+    e = hipMemcpyAsync(chunks[i].dst, p + o, chunks[i].bytes, hipMemcpyHostToDevice, s);
     o += align16(chunks[i].bytes);
   }
-  RPX_TRY(hipEventRecord(slot.done, s));
+  if (e == hipSuccess) e = hipEventRecord(slot.done, s);
+  if (e != hipSuccess) {
+    // Copies already queued may still read the slot: let them finish
+    // before it can be reused.
+    (void)hipStreamSynchronize(s);
+    return hip_status(e);
+  }
   slot.pending = true;
   return RPX_SUCCESS;
 }
