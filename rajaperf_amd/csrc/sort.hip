@@ -131,8 +131,11 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* w
 }
 
 // One workgroup per segment: exclusive scan of counts in digit-major order.
-// Work-item d owns digit d; the tiles are visited in batches of 8 so that 8
-// coalesced loads are in flight at a time.
+// Work-item d owns digit d; the tiles are visited in batches of kBatch so
+// that kBatch coalesced loads are in flight at a time (32 against 8: the
+// 256-segment ragged sort 2.14 -> 2.09 ms, the largest segment's workgroup
+// setting the kernel's time).
+constexpr int kBatch = 32;
 __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ counts, Plan plan) {
   __shared__ unsigned wsum[kBS / kWave];
   long long base;
@@ -148,21 +151,21 @@ __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ cou
   unsigned* col = counts + base + d;  // col[t * kRadix]: digit d of tile t
   unsigned s = 0;
   int t = 0;
-  for (; t + 8 <= nt; t += 8) {
-    unsigned v[8];
+  for (; t + kBatch <= nt; t += kBatch) {
+    unsigned v[kBatch];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = col[(long long)(t + u) * kRadix];
+    for (int u = 0; u < kBatch; ++u) v[u] = col[(long long)(t + u) * kRadix];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u];
+    for (int u = 0; u < kBatch; ++u) s += v[u];
   }
   for (; t < nt; ++t) s += col[(long long)t * kRadix];
   unsigned run = block_exclusive_scan(s, wsum);
-  for (t = 0; t + 8 <= nt; t += 8) {
-    unsigned v[8];
+  for (t = 0; t + kBatch <= nt; t += kBatch) {
+    unsigned v[kBatch];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = col[(long long)(t + u) * kRadix];
+    for (int u = 0; u < kBatch; ++u) v[u] = col[(long long)(t + u) * kRadix];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < kBatch; ++u) {
       col[(long long)(t + u) * kRadix] = run;
       run += v[u];
     }
