@@ -131,13 +131,19 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* w
 }
 
 // One workgroup per segment: exclusive scan of counts in digit-major order.
-// Work-item d owns digit d; the tiles are visited in batches of kBatch so
-// that kBatch coalesced loads are in flight at a time (32 against 8: the
-// 256-segment ragged sort 2.14 -> 2.09 ms, the largest segment's workgroup
-// setting the kernel's time).
+// kWalkers work-items share each digit: walker w of digit d visits the
+// segment's tiles [w*q, (w+1)*q), q = ceil(nt / kWalkers), in batches of
+// kBatch coalesced loads.  The largest segment's workgroup sets the kernel's
+// time, and its walk is kWalkers times shorter than with one work-item per
+// digit.  The walkers' sums give each digit's total (scanned over the
+// digits) and each walker's starting offset; the walkers then write their
+// tiles' offsets.
 constexpr int kBatch = 32;
-__global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ counts, Plan plan) {
-  __shared__ unsigned wsum[kBS / kWave];
+constexpr int kWalkers = 4;
+constexpr int kOffThreads = kWalkers * kRadix;
+__global__ __launch_bounds__(kOffThreads) void offsets_kernel(unsigned* __restrict__ counts, Plan plan) {
+  __shared__ unsigned part[kWalkers][kRadix];
+  __shared__ unsigned wsum[kRadix / kWave];
   long long base;
   int nt;
   if (plan.segs) {
@@ -147,20 +153,52 @@ __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ cou
     base = (long long)blockIdx.x * plan.tiles_per_seg * kRadix;
     nt = plan.tiles_per_seg;
   }
-  const int d = threadIdx.x;
+  const int d = threadIdx.x % kRadix, w = threadIdx.x / kRadix;  // w is wavefront-uniform
+  const int q = (nt + kWalkers - 1) / kWalkers;
+  const int t0 = w * q < nt ? w * q : nt;
+  const int t1 = (w + 1) * q < nt ? (w + 1) * q : nt;
   unsigned* col = counts + base + d;  // col[t * kRadix]: digit d of tile t
   unsigned s = 0;
-  int t = 0;
-  for (; t + kBatch <= nt; t += kBatch) {
+  int t = t0;
+  for (; t + kBatch <= t1; t += kBatch) {
     unsigned v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) v[u] = col[(long long)(t + u) * kRadix];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) s += v[u];
   }
-  for (; t < nt; ++t) s += col[(long long)t * kRadix];
-  unsigned run = block_exclusive_scan(s, wsum);
-  for (t = 0; t + kBatch <= nt; t += kBatch) {
+  for (; t < t1; ++t) s += col[(long long)t * kRadix];
+  part[w][d] = s;
+  __syncthreads();
+  // Walker 0's wavefronts: digit totals, scanned over the 256 digits.
+  unsigned run = 0;
+  if (w == 0) {
+    unsigned total = 0;
+#pragma unroll
+    for (int k = 0; k < kWalkers; ++k) total += part[k][d];
+    const int lane = d & (kWave - 1), wave = d / kWave;
+    unsigned incl = total;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const unsigned o = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += o;
+    }
+    if (lane == kWave - 1) wsum[wave] = incl;
+    run = incl - total;
+  }
+  __syncthreads();
+  if (w == 0) {
+    for (int k = 0; k < d / kWave; ++k) run += wsum[k];
+#pragma unroll
+    for (int k = 0; k < kWalkers; ++k) {  // each walker's first offset
+      const unsigned c = part[k][d];
+      part[k][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  run = part[w][d];
+  for (t = t0; t + kBatch <= t1; t += kBatch) {
     unsigned v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) v[u] = col[(long long)(t + u) * kRadix];
@@ -170,7 +208,7 @@ __global__ __launch_bounds__(kBS) void offsets_kernel(unsigned* __restrict__ cou
       run += v[u];
     }
   }
-  for (; t < nt; ++t) {
+  for (; t < t1; ++t) {
     const unsigned c = col[(long long)t * kRadix];
     col[(long long)t * kRadix] = run;
     run += c;
@@ -394,7 +432,7 @@ int rpx_sort_segmented_f64(double* x, const int64_t* offsets_host, int64_t nseg,
     const int shift = 8 * pass;
     const int src_is_x = (pass % 2 == 0);
     hist_kernel<<<dim3(unsigned(s.total_tiles)), dim3(kBS), 0, st>>>(src, counts, plan, shift, pass == 0, src_is_x);
-    offsets_kernel<<<dim3(unsigned(nseg_launch)), dim3(kBS), 0, st>>>(counts, plan);
+    offsets_kernel<<<dim3(unsigned(nseg_launch)), dim3(kOffThreads), 0, st>>>(counts, plan);
     scatter_kernel<<<dim3(unsigned(s.total_tiles)), dim3(kBS), 0, st>>>(src, dst, counts, plan, shift, pass == 0,
                                                                         pass == kPasses - 1, src_is_x);
     const int rc = launch_status();
