@@ -189,7 +189,9 @@ __global__ void zero_u32_kernel(unsigned* p, int64_t n) {
 // Defaults from a sweep on MI355X at 2^28 doubles: 256 work-items, 4
 // vectors per step, 4 steps (8192 doubles per workgroup for both DOT and
 // REDUCE_SUM).
-constexpr int kDefaultBlock = 256;
+// 512 work-items per workgroup: against 256, DOT 0.628 -> 0.619 ms and
+// REDUCE_SUM 0.317 -> 0.311 ms at 2^28 doubles (interleaved, six repeats).
+constexpr int kDefaultBlock = 512;
 constexpr int kDefaultItems = 32;  // doubles per work-item over its workgroup's chunk
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
