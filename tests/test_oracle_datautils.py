@@ -8,6 +8,7 @@ import math
 from fractions import Fraction
 
 import numpy as np
+import pytest
 
 
 def test_splitmix64_published_vectors(orc):
@@ -122,3 +123,59 @@ def test_fem_basis_is_lagrange_and_symmetric(orc):
                 assert np.allclose(Bm @ xd ** k, xq ** k, atol=1e-11)
                 dexp = k * xq ** (k - 1) if k > 0 else np.zeros(q1d)
                 assert np.allclose(Gm @ xd ** k, dexp, atol=1e-9)
+
+
+def _splitmix64_py(state, count):
+    """SplitMix64 (Steele, Lea & Flood 2014), written from the published
+    algorithm; the published vectors above pin the oracle's version too."""
+    M = (1 << 64) - 1
+    out = []
+    for _ in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & M
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def _mix64_py(z):
+    M = (1 << 64) - 1
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+@pytest.mark.parametrize("sym", [6, 9])
+def test_fill_sym_tensor_slots_and_ranges(orc, sym):
+    # Component names give the diagonal slots: (11,12,13,22,23,33) for the
+    # six-component half storage (DIFFUSION3DPA.hpp:384-392), row-major 3x3
+    # for nine.  Diagonal entries are uniform in [0.5, 2], the others in
+    # [-0.25, 0.25], each from the SplitMix64 value at its flat index.
+    comps = ([(1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)] if sym == 6
+             else [(r, c) for r in (1, 2, 3) for c in (1, 2, 3)])
+    diag = np.array([r == c for r, c in comps])
+    ne, q3, seed, sid = 3, 8, 42, 7
+    t = orc.fill_sym_tensor(ne, q3, sym, seed, sid)
+    n = ne * sym * q3
+    state0 = _mix64_py(seed ^ ((sid * 0xD1B54A32D192ED03) & ((1 << 64) - 1)))
+    u = np.array([(s >> 11) * 2.0 ** -53 for s in _splitmix64_py(state0, n)])
+    slot = (np.arange(n) // q3) % sym
+    lo = np.where(diag[slot], 0.5, -0.25)
+    width = np.where(diag[slot], 1.5, 0.5)
+    assert np.array_equal(t, lo + width * u)
+    # the same values as the plain uniform generator over the flat index
+    uu = orc.fill_uniform(n, seed, sid, 0.0, 1.0)
+    assert np.array_equal(u, uu)
+    # ranges per slot on a large sample: the extremes approach both ends
+    big = orc.fill_sym_tensor(200, 125, sym, seed, sid).reshape(200, sym, 125)
+    for s_, is_diag in enumerate(diag):
+        v = big[:, s_, :]
+        lo_, hi_ = (0.5, 2.0) if is_diag else (-0.25, 0.25)
+        assert v.min() >= lo_ and v.max() <= hi_, (s_, v.min(), v.max())
+        assert v.min() < lo_ + 0.01 * (hi_ - lo_) and v.max() > hi_ - 0.01 * (hi_ - lo_)
+    # a symmetric field of the nine-component layout: diagonal slots 0, 4, 8
+    if sym == 9:
+        assert [i for i, d in enumerate(diag) if d] == [0, 4, 8]
+    else:
+        assert [i for i, d in enumerate(diag) if d] == [0, 3, 5]

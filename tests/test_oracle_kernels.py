@@ -216,6 +216,52 @@ def test_diffusion3dpa_symmetric_bases_equal_textbook(orc, rng, v, d1d):
                for e in range(NE))
 
 
+def _diffusion_dense_full(Bm, Gm, D9, X, NE, d1d, q1d):
+    """Textbook Y_e = sum_r sum_c G_r^T O_rc G_c X_e with O the full 3x3 tensor
+    of each quadrature point given row-major (component 3r+c is O_{r+1,c+1}),
+    G_x = B (x) B (x) G' etc.; nothing assumes O symmetric."""
+    Y = np.zeros(NE * d1d ** 3)
+    for e in range(NE):
+        Xe = X[e * d1d ** 3:(e + 1) * d1d ** 3].reshape(d1d, d1d, d1d)  # [z,y,x]
+        O = D9[e * 9 * q1d ** 3:(e + 1) * 9 * q1d ** 3].reshape(3, 3, q1d, q1d, q1d)
+        grads = [np.einsum("ZD,YE,XF,DEF->ZYX", Bm, Bm, Gm, Xe),   # d/dx
+                 np.einsum("ZD,YE,XF,DEF->ZYX", Bm, Gm, Bm, Xe),   # d/dy
+                 np.einsum("ZD,YE,XF,DEF->ZYX", Gm, Bm, Bm, Xe)]   # d/dz
+        flux = [sum(O[r, c] * grads[c] for c in range(3)) for r in range(3)]
+        Ye = (np.einsum("ZD,YE,XF,ZYX->DEF", Bm, Bm, Gm, flux[0])
+              + np.einsum("ZD,YE,XF,ZYX->DEF", Bm, Gm, Bm, flux[1])
+              + np.einsum("ZD,YE,XF,ZYX->DEF", Gm, Bm, Bm, flux[2]))
+        Y[e * d1d ** 3:(e + 1) * d1d ** 3] = Ye.reshape(-1)
+    return Y
+
+
+@pytest.mark.parametrize("v", VARIANTS)
+@pytest.mark.parametrize("d1d", [2, 3, 4])
+def test_diffusion3dpa_full_tensor_equals_textbook(orc, rng, v, d1d):
+    # symmetric=0 reads nine components per point (DIFFUSION3DPA.hpp:384-392).
+    # A tensor with independent entries in every slot, O12 != O21 etc., makes
+    # a transposed, swapped or symmetrised component read change Y.
+    q1d = d1d + 1
+    NE = 2
+    b, g = orc.fem_basis(d1d, q1d)
+    D9 = rng.uniform(-1, 1, NE * 9 * q1d ** 3)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    Y = orc.diffusion3dpa(b, g, D9, X, np.zeros_like(X), NE, d1d, q1d, symmetric=False, variant=v)
+    Bm = b.reshape(d1d, q1d).T
+    Gm = g.reshape(d1d, q1d).T
+    ref = _diffusion_dense_full(Bm, Gm, D9, X, NE, d1d, q1d)
+    assert np.allclose(Y, ref, rtol=0, atol=1e-11)
+    # the transposed tensor is a different operator here, so the check binds
+    D9t = D9.reshape(NE, 3, 3, -1).transpose(0, 2, 1, 3).reshape(-1)
+    assert not np.allclose(ref, _diffusion_dense_full(Bm, Gm, D9t, X, NE, d1d, q1d), rtol=0, atol=1e-6)
+    # a symmetric 9-component tensor gives the 6-component result
+    Dsym = 0.5 * (D9 + D9t)
+    D6 = Dsym.reshape(NE, 9, -1)[:, [0, 1, 2, 4, 5, 8], :].reshape(-1)
+    Y9 = orc.diffusion3dpa(b, g, Dsym, X, np.zeros_like(X), NE, d1d, q1d, symmetric=False, variant=v)
+    Y6 = orc.diffusion3dpa(b, g, D6, X, np.zeros_like(X), NE, d1d, q1d, symmetric=True, variant=v)
+    assert np.allclose(Y9, Y6, rtol=0, atol=1e-12)
+
+
 @pytest.mark.parametrize("d1d", [3, 4])
 def test_diffusion3dpa_all_ones_follows_seq_write_order(orc, rng, d1d):
     q1d = d1d + 1
