@@ -185,7 +185,10 @@ int rpx_reduce_sum_f64(const double* x, int64_t n, double* result,
  * The scratch size does not depend on the tuning.  x and y may alias. */
 size_t rpx_scan_scratch_bytes(int64_t n, const rpx_tuning* tune);
 /* Zeroes a scan scratch buffer; required once before its first use.  Each
- * call then leaves the buffer ready for the next (epoch-tagged flags). */
+ * call then leaves the buffer ready for the next (epoch-tagged flags).  A
+ * scratch may be shared with the segmented form below (sized for the larger
+ * of the two): the segmented form's flags carry a tag the single-array form
+ * never accepts. */
 int rpx_scan_scratch_init(void* scratch, size_t scratch_bytes, void* stream);
 /* Synchronously reads the scratch's error word: RPX_ERR_WAIT_EXPIRED if a
  * look-back wait of an earlier call gave up, else RPX_SUCCESS. */
@@ -273,7 +276,13 @@ int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha,
 /* Mixed-precision GEMM, same semantics: A and B are split into `nslices`
  * error-free fixed-point slices, slice products run on MFMA with exact
  * integer accumulation, and the result is recombined in FP64.  nslices in
- * [1, 8]; 0 = default (7, error below 1e-12 relative to |alpha||A||B|). */
+ * [1, 8]; 0 = default 7.  Rows of A and columns of B are scaled by powers
+ * of two 2^e_i >= max_k |A[i][k]| and 2^f_j >= max_k |B[k][j]|; the error of
+ * C[i][j] is below (s+3) 2^-7s |alpha| 2^(e_i+f_j) nk (1.8e-14 |alpha|
+ * 2^(e_i+f_j) nk at s = 7), so relative to |alpha|(|A||B|)_ij it grows
+ * with the dynamic range within row i and column j.  nk is limited to
+ * (2^31-1)/(s 127^2) (19020 at s = 7); larger nk returns
+ * RPX_ERR_UNSUPPORTED. */
 size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk,
                                     int nslices);
 int rpx_gemm_mixed_f64(double* C, const double* A, const double* B,
@@ -364,6 +373,29 @@ int rpx_diffusion3dpa_f64(const double* Basis, const double* dBasis,
                           const double* D, const double* X, double* Y,
                           int64_t NE, int d1d, int q1d, int symmetric,
                           const rpx_tuning* tune, void* stream);
+
+/* The same operator with its basis contractions on int8 matrix cores: the
+ * forward and backward products with the three Kronecker matrices
+ * K_0 = B(x)B(x)G, K_1 = B(x)G(x)B, K_2 = G(x)B(x)B (built from the same
+ * effective half-storage tables, so the result is the Base_Seq operator for
+ * any bases) run as error-free int8 slice products (`nslices` digits of 7
+ * bits, 5..8, 0 = 7) with int32 accumulation and FP64 recombination; the
+ * flux O.grad at each point and Y += are FP64.  Rows of K_c and K_c^T, and
+ * X and the flux per element, are scaled by powers of two; each product's
+ * dropped digits are below (nslices+3) 2^(-7 nslices) times its operands'
+ * scales times its length, and at nslices = 7 every output is within 1e-12
+ * of the FP64 result relative to the magnitude operator
+ * sum_c |K_c|^T (|O| (|K| |X|))_c on the test data (tests/test_gpu_gemm_mixed.py).
+ * No counterpart in the suite (the mixed-precision path of
+ * DIFFUSION3DPA-Hip.cpp's contractions).  Supported: 1 <= d1d <= 4,
+ * q1d = d1d+1.  `scratch` (query rpx_diffusion3dpa_mixed_scratch_bytes)
+ * receives the sliced Kronecker matrices; it needs no initialisation. */
+size_t rpx_diffusion3dpa_mixed_scratch_bytes(int d1d, int q1d, int nslices);
+int rpx_diffusion3dpa_mixed_f64(const double* Basis, const double* dBasis,
+                                const double* D, const double* X, double* Y,
+                                int64_t NE, int d1d, int q1d, int symmetric,
+                                int nslices, void* scratch,
+                                size_t scratch_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,17 @@ void orc_split_ld(long double v, double* hi, double* lo) {
   *lo = static_cast<double>(v - static_cast<long double>(*hi));
 }
 
+// The i-weighted FP64 sum sum_i (i+1)*x[i], summed in index order with no
+// fused multiply-add: a second per-kernel checksum, the "sum of i-weighted
+// elements in FP64" of the stream workload's description.  Weights start at
+// 1 (as the suite's |sin(j+1)| weights do) so that a scalar result (DOT,
+// REDUCE_SUM) contributes itself.
+double orc_iweighted_sum(const double* p, int64_t n) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += static_cast<double>(i + 1) * p[i];
+  return s;
+}
+
 void orc_checksum_parts(const double* p, int64_t n, double scale, double* hi, double* lo) {
   orc_split_ld(orc_checksum(p, n, scale), hi, lo);
 }

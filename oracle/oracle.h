@@ -44,6 +44,8 @@ void orc_init_data_rand_value(double* p, int64_t n);      /* :560-569 */
 void orc_init_scalar(double* d);                          /* :589-595 */
 long double orc_checksum(const double* p, int64_t n, double scale); /* :601-621 */
 long double orc_checksum_int(const int* p, int64_t n, double scale);
+/* sum_i (i+1)*x[i] in FP64, index order, no FMA (the i-weighted checksum). */
+double orc_iweighted_sum(const double* p, int64_t n);
 /* Exact two-double split (hi + lo) of a long double / of a checksum. */
 void orc_split_ld(long double v, double* hi, double* lo);
 void orc_checksum_parts(const double* p, int64_t n, double scale, double* hi, double* lo);
@@ -100,6 +102,8 @@ typedef struct orc_result {
   int64_t problem_size;
   int64_t bytes_per_rep;
   int64_t flops_per_rep;
+  double iweighted;     /* i-weighted sums of the checksummed arrays, summed
+                           over arrays and passes                          */
 } orc_result;
 
 /* kernel: "Stream_TRIAD", ...; variant: "Base_Seq" or "Base_OpenMP".

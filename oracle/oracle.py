@@ -28,7 +28,8 @@ class Params(C.Structure):
 class Result(C.Structure):
     _fields_ = [("checksum", C.c_longdouble), ("checksum_hi", C.c_double), ("checksum_lo", C.c_double),
                 ("time_total_s", C.c_double), ("reps", C.c_int64),
-                ("problem_size", C.c_int64), ("bytes_per_rep", C.c_int64), ("flops_per_rep", C.c_int64)]
+                ("problem_size", C.c_int64), ("bytes_per_rep", C.c_int64), ("flops_per_rep", C.c_int64),
+                ("iweighted", C.c_double)]
 
 
 def build() -> None:
@@ -51,7 +52,7 @@ def lib():
             "orc_init_data": (None, [P, i64]), "orc_init_data_int": (None, [P, i64]),
             "orc_init_data_const": (None, [P, i64, d]), "orc_init_data_rand_sign": (None, [P, i64]),
             "orc_init_data_rand_value": (None, [P, i64]), "orc_init_scalar": (None, [dp]),
-            "orc_checksum": (C.c_longdouble, [P, i64, d]),
+            "orc_checksum": (C.c_longdouble, [P, i64, d]), "orc_iweighted_sum": (d, [P, i64]),
             "orc_checksum_parts": (None, [P, i64, d, dp, dp]), "orc_checksum_int": (C.c_longdouble, [P, i64, d]),
             "orc_splitmix64_next": (u64, [C.POINTER(u64)]),
             "orc_fill_uniform": (None, [P, i64, u64, u64, d, d]),
@@ -138,6 +139,11 @@ def checksum_exact(a: np.ndarray, scale: float = 1.0):
     hi, lo = C.c_double(), C.c_double()
     lib().orc_checksum_parts(_p(a), a.size, scale, C.byref(hi), C.byref(lo))
     return Fraction(hi.value) + Fraction(lo.value)
+
+
+def iweighted_sum(a: np.ndarray) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return lib().orc_iweighted_sum(_p(a), a.size)
 
 
 def checksum_int(a: np.ndarray, scale: float = 1.0) -> float:
@@ -262,4 +268,4 @@ def run(kernel: str, variant: str = "Base_Seq", *, input_mode: int = 0, seed: in
         raise ValueError(f"oracle has no {variant} for {kernel}")
     return {"checksum": float(r.checksum), "checksum_hi": r.checksum_hi, "checksum_lo": r.checksum_lo, "time_s": r.time_total_s,
             "reps": r.reps, "problem_size": r.problem_size, "bytes_per_rep": r.bytes_per_rep,
-            "flops_per_rep": r.flops_per_rep}
+            "flops_per_rep": r.flops_per_rep, "iweighted": r.iweighted}

@@ -64,6 +64,14 @@ struct Timer {
   void stop() { total += std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
+// The suite checksum of an array, with the array's i-weighted sum added to
+// the pass's running total.
+double g_iweighted = 0.0;
+long double checksum(const double* p, int64_t n, double scale) {
+  g_iweighted += orc_iweighted_sum(p, n);
+  return orc_checksum(p, n, scale);
+}
+
 int variant_id(const char* v) {
   if (std::strcmp(v, "Base_Seq") == 0) return 0;
   if (std::strcmp(v, "Base_OpenMP") == 0) return 1;
@@ -87,7 +95,7 @@ long double run_triad(const Ctx& c, int vid, orc_result* r, Timer& t) {
   for (int64_t irep = 0; irep < reps; ++irep) orc_triad(vid, a.data(), b.data(), cc.data(), alpha, n);
   t.stop();
   const long double scale = 0.001 * (static_cast<long double>(1000000) / n);  // TRIAD.cpp:36-38
-  return orc_checksum(a.data(), n, static_cast<double>(scale));
+  return checksum(a.data(), n, static_cast<double>(scale));
 }
 
 long double run_copy(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -102,7 +110,7 @@ long double run_copy(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.start();
   for (int64_t irep = 0; irep < reps; ++irep) orc_copy(vid, cc.data(), a.data(), n);
   t.stop();
-  return orc_checksum(cc.data(), n, 1.0);
+  return checksum(cc.data(), n, 1.0);
 }
 
 long double run_mul(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -118,7 +126,7 @@ long double run_mul(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.start();
   for (int64_t irep = 0; irep < reps; ++irep) orc_mul(vid, b.data(), cc.data(), alpha, n);
   t.stop();
-  return orc_checksum(b.data(), n, 1.0);
+  return checksum(b.data(), n, 1.0);
 }
 
 long double run_add(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -134,7 +142,7 @@ long double run_add(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.start();
   for (int64_t irep = 0; irep < reps; ++irep) orc_add(vid, cc.data(), a.data(), b.data(), n);
   t.stop();
-  return orc_checksum(cc.data(), n, 1.0);
+  return checksum(cc.data(), n, 1.0);
 }
 
 long double run_dot(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -150,7 +158,8 @@ long double run_dot(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.start();
   for (int64_t irep = 0; irep < reps; ++irep) m_dot += orc_dot(vid, a.data(), b.data(), n);  // DOT-Seq.cpp:39-45
   t.stop();
-  return m_dot;  // DOT.cpp:78-81
+  g_iweighted += m_dot;  // a scalar's i-weighted sum is itself
+  return m_dot;         // DOT.cpp:78-81
 }
 
 // ---- algorithm (src/algorithm/*.cpp) ----------------------------------------
@@ -167,7 +176,7 @@ long double run_reduce_sum(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.start();
   for (int64_t irep = 0; irep < reps; ++irep) m_sum = orc_reduce_sum(vid, x.data(), n);  // -Seq.cpp:38-45
   t.stop();
-  return orc_checksum(&m_sum, 1, 1.0);  // REDUCE_SUM.cpp:73-76
+  return checksum(&m_sum, 1, 1.0);  // REDUCE_SUM.cpp:73-76
 }
 
 long double run_scan(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -184,7 +193,7 @@ long double run_scan(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.stop();
   const long double scale =
       1e-2 * (static_cast<long double>(1000000) / n) / n;  // SCAN.cpp:36-39
-  return orc_checksum(y.data(), n, static_cast<double>(scale));
+  return checksum(y.data(), n, static_cast<double>(scale));
 }
 
 long double run_sort(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -199,7 +208,7 @@ long double run_sort(const Ctx& c, int vid, orc_result* r, Timer& t) {
   t.start();
   for (int64_t irep = 0; irep < reps; ++irep) orc_sort(x.data() + n * irep, n);  // SORT-Seq.cpp:31-42
   t.stop();
-  return orc_checksum(x.data(), n * reps, 1.0);
+  return checksum(x.data(), n * reps, 1.0);
 }
 
 // ---- polybench ----------------------------------------------------------------
@@ -224,7 +233,7 @@ long double run_gemm(const Ctx& c, int vid, orc_result* r, Timer& t) {
   for (int64_t irep = 0; irep < reps; ++irep) orc_gemm(vid, C.data(), A.data(), B.data(), alpha, beta, ni, nj, nk);
   t.stop();
   const long double scale = 0.001 * (static_cast<long double>(1000 * 1000) / (ni * nj));  // :50-52
-  return orc_checksum(C.data(), ni * nj, static_cast<double>(scale));
+  return checksum(C.data(), ni * nj, static_cast<double>(scale));
 }
 
 void seeded_grid(std::vector<double>& g, int64_t N, uint64_t seed, int sid) {
@@ -260,8 +269,8 @@ long double run_jacobi2d(const Ctx& c, int vid, orc_result* r, Timer& t) {
   for (int64_t irep = 0; irep < reps; ++irep) orc_jacobi2d(vid, A.data(), B.data(), N, tsteps);
   t.stop();
   const long double scale = 0.0001 * (static_cast<long double>(default_size) / ((N - 2) * (N - 2)));
-  long double cs = orc_checksum(A.data(), N * N, static_cast<double>(scale));
-  cs += orc_checksum(B.data(), N * N, static_cast<double>(scale));  // :91-95
+  long double cs = checksum(A.data(), N * N, static_cast<double>(scale));
+  cs += checksum(B.data(), N * N, static_cast<double>(scale));  // :91-95
   return cs;
 }
 
@@ -302,7 +311,7 @@ long double run_mass3dpa(const Ctx& c, int vid, orc_result* r, Timer& t) {
   for (int64_t irep = 0; irep < reps; ++irep)
     orc_mass3dpa(vid, B.data(), Bt.data(), D.data(), X.data(), Y.data(), NE, D1, Q1);
   t.stop();
-  return orc_checksum(Y.data(), D3 * NE, 1.0);  // MASS3DPA.cpp:86-89
+  return checksum(Y.data(), D3 * NE, 1.0);  // MASS3DPA.cpp:86-89
 }
 
 long double run_diffusion3dpa(const Ctx& c, int vid, orc_result* r, Timer& t) {
@@ -341,7 +350,7 @@ long double run_diffusion3dpa(const Ctx& c, int vid, orc_result* r, Timer& t) {
   for (int64_t irep = 0; irep < reps; ++irep)
     orc_diffusion3dpa(vid, B.data(), G.data(), D.data(), X.data(), Y.data(), NE, D1, Q1, 1);
   t.stop();
-  return orc_checksum(Y.data(), D3 * NE, 1.0);  // DIFFUSION3DPA.cpp:90-93
+  return checksum(Y.data(), D3 * NE, 1.0);  // DIFFUSION3DPA.cpp:90-93
 }
 
 using RunFn = long double (*)(const Ctx&, int, orc_result*, Timer&);
@@ -375,6 +384,7 @@ extern "C" int orc_run(const char* kernel, const char* variant, const orc_params
   std::memset(out, 0, sizeof(*out));
   long double cs = 0.0;
   const int npasses = p->npasses > 0 ? p->npasses : 1;
+  g_iweighted = 0.0;
   for (int pass = 0; pass < npasses; ++pass) {
     orc_reset_init_count();  // KernelBase.cpp:366
     cs += fn(c, vid, out, t);
@@ -382,5 +392,6 @@ extern "C" int orc_run(const char* kernel, const char* variant, const orc_params
   out->checksum = cs;
   orc_split_ld(cs, &out->checksum_hi, &out->checksum_lo);
   out->time_total_s = t.total;
+  out->iweighted = g_iweighted;
   return 0;
 }

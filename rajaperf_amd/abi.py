@@ -121,6 +121,8 @@ _SIGS = {
     "rpx_mass3dpa_mixed_scratch_bytes": (_sz, [_i32, _i32, _i32]),
     "rpx_mass3dpa_mixed_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp, _sz, _vp]),
     "rpx_diffusion3dpa_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _tp, _vp]),
+    "rpx_diffusion3dpa_mixed_scratch_bytes": (_sz, [_i32, _i32, _i32]),
+    "rpx_diffusion3dpa_mixed_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _vp, _sz, _vp]),
 }
 
 # Entry points whose return value is a size, not a status.

@@ -32,6 +32,7 @@
 // through L2 (loading them inside the slice loop left the MFMAs waiting on
 // L2 latency: 0.58 ms at NE = 262144).
 #include "rpx_common.hpp"
+#include "fem_common.hpp"
 
 using namespace rpx;
 
@@ -407,6 +408,322 @@ int run_slices(const double* B, const double* Bt, const double* Dq, const double
   }
 }
 
+// ---- DIFFUSION3DPA ------------------------------------------------------------
+//
+// The diffusion operator (src/apps/DIFFUSION3DPA.hpp:216-458,
+// DIFFUSION3DPA-Seq.cpp:28-110) in Kronecker form.  With the effective
+// half-storage tables Be(q,d), Ge(q,d) of diffusion_tables (so that bases
+// breaking the MFEM symmetry give the Base_Seq operator), element e's
+// gradient at the quadrature points is
+//   T_c = K_c X_e,  K_0 = Be(x)Be(x)Ge,  K_1 = Be(x)Ge(x)Be,  K_2 = Ge(x)Be(x)Be
+// (factors for z, y, x: K_c(m, k) with m = (qz*Q + qy)*Q + qx and
+// k = (dz*D + dy)*D + dx), the flux is U_c = sum_c' O_cc' T_c' at every
+// point (O symmetric, stored as (11,12,13,22,23,33), or nine components
+// row-major, DIFFUSION3DPA.hpp:384-392), and Y_e += sum_c K_c^T U_c.
+// Forward and backward products are the MASS3DPA scheme above, per
+// component: rows of K_c and of K_c^T scaled by powers of two, X and U per
+// element, S digits each, exact int32 levels, FP64 recombination; the flux
+// and Y += are FP64.
+//
+// A workgroup of four wavefronts owns 16 elements.  Wavefront w computes the
+// quadrature rows [32w, 32w + 32) of all three components (so the flux at a
+// point needs no exchange), then the dof rows [16w, 16w + 16) of the
+// backward product.  The slices of K_c (forward) and of K_c^T (backward) of
+// the wavefront's rows are loaded into registers one component at a time.
+// LDS holds X's slices, then U's slices for the three components (S x 16 x
+// 432 bytes, 48 KB at S = 7).  O is read straight from global memory in
+// the accumulator layout: 16 consecutive points of one component and
+// element per quarter-wavefront, 128 contiguous bytes.
+constexpr int kDEB = 16;                // elements per workgroup
+constexpr int kDUComp = kQP + 16;       // LDS bytes per component of an element's U slice row
+constexpr int kDURow = 3 * kDUComp;
+
+struct DiffMixedLayout {
+  size_t kf_off, kb_off, ea_off, eb_off, total;
+};
+DiffMixedLayout diff_layout(int s) {
+  DiffMixedLayout l;
+  l.kf_off = 0;                                          // Kf[p][c][m][k]
+  l.kb_off = align256(size_t(s) * 3 * kQP * kDP);        // Kb[p][c][j][m]
+  l.ea_off = l.kb_off + align256(size_t(s) * 3 * kDP * kQP);
+  l.eb_off = l.ea_off + align256(3 * kQP * sizeof(int));  // ea[c][m]
+  l.total = l.eb_off + align256(3 * kDP * sizeof(int));   // eb[c][j]
+  return l;
+}
+
+// One workgroup per component c: work-items 0..127 cut the rows of K_c,
+// 128..191 the rows of K_c^T.
+template <int S>
+__global__ __launch_bounds__(256) void diff_kron_slices_kernel(const double* __restrict__ Bg,
+                                                               const double* __restrict__ Gg, int D, int Q,
+                                                               signed char* __restrict__ Kf,
+                                                               signed char* __restrict__ Kb, int* __restrict__ ea,
+                                                               int* __restrict__ eb) {
+  __shared__ double sB[64], sG[64];  // effective tables, [q*D + d]
+  const int c = blockIdx.x, r = threadIdx.x;
+  const int D3 = D * D * D, Q3 = Q * Q * Q;
+  if (r < Q * D) diffusion_tables(r / D, r % D, Q, D, Bg, Gg, &sB[r], &sG[r]);
+  __syncthreads();
+  // factor of axis a (0 = z, 1 = y, 2 = x): Ge on axis 2 - c, Be elsewhere
+  auto f = [&](int a, int q, int d) { return a == 2 - c ? sG[q * D + d] : sB[q * D + d]; };
+  auto kval = [&](int m, int k) {
+    const int qz = m / (Q * Q), qy = (m / Q) % Q, qx = m % Q;
+    const int dz = k / (D * D), dy = (k / D) % D, dx = k % D;
+    return (f(0, qz, dz) * f(1, qy, dy)) * f(2, qx, dx);
+  };
+  if (r < kQP) {
+    const int m = r;
+    double mx = 0.0;
+    if (m < Q3)
+      for (int k = 0; k < D3; ++k) mx = fmax(mx, fabs(kval(m, k)));
+    const int e = exponent_of(mx);
+    ea[c * kQP + m] = e;
+    for (int k = 0; k < kDP; k += 4) {
+      int dig[4][S];
+      for (int u = 0; u < 4; ++u) {
+        const int kk = k + u;
+        cut<S>(ldexp((m < Q3 && kk < D3) ? kval(m, kk) : 0.0, -e), dig[u]);
+      }
+      for (int p = 0; p < S; ++p)
+        *reinterpret_cast<int*>(Kf + ((size_t(p) * 3 + c) * kQP + m) * kDP + k) =
+            pack4(dig[0][p], dig[1][p], dig[2][p], dig[3][p]);
+    }
+  } else if (r < kQP + kDP) {
+    const int j = r - kQP;
+    double mx = 0.0;
+    if (j < D3)
+      for (int m = 0; m < Q3; ++m) mx = fmax(mx, fabs(kval(m, j)));
+    const int e = exponent_of(mx);
+    eb[c * kDP + j] = e;
+    for (int m = 0; m < kQP; m += 4) {
+      int dig[4][S];
+      for (int u = 0; u < 4; ++u) {
+        const int mm = m + u;
+        cut<S>(ldexp((j < D3 && mm < Q3) ? kval(mm, j) : 0.0, -e), dig[u]);
+      }
+      for (int p = 0; p < S; ++p)
+        *reinterpret_cast<int*>(Kb + ((size_t(p) * 3 + c) * kDP + j) * kQP + m) =
+            pack4(dig[0][p], dig[1][p], dig[2][p], dig[3][p]);
+    }
+  }
+}
+
+template <int D, int Q, int S, bool SYMM>
+__global__ __launch_bounds__(256) void diffusion3dpa_mixed_kernel(const signed char* __restrict__ Kf,
+                                                                  const signed char* __restrict__ Kb,
+                                                                  const int* __restrict__ ea,
+                                                                  const int* __restrict__ eb,
+                                                                  const double* __restrict__ Dg,
+                                                                  const double* __restrict__ Xg,
+                                                                  double* __restrict__ Yg, int64_t NE) {
+  constexpr int D3 = D * D * D, Q3 = Q * Q * Q;
+  constexpr int NC = SYMM ? 6 : 9;
+  constexpr int kXsBytes = S * kDEB * kXRow;
+  constexpr int kUsBytes = S * kDEB * kDURow;
+  constexpr int kLdsBytes = kXsBytes > kUsBytes ? kXsBytes : kUsBytes;
+  __shared__ __attribute__((aligned(16))) signed char lds[kLdsBytes];
+  __shared__ int eX[kDEB];
+  __shared__ double umax[4][kDEB];
+  auto Xs = reinterpret_cast<signed char(*)[kDEB][kXRow]>(lds);
+  auto Us = reinterpret_cast<signed char(*)[kDEB][kDURow]>(lds);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int fr = lane & 15, fk = (lane >> 4) * 16;  // MFMA operand row (element) / first K byte
+  const int64_t e0 = int64_t(blockIdx.x) * kDEB;
+
+  // X -> exponent and slices: sixteen work-items per element, four values each.
+  {
+    const int n = tid >> 4, k0 = (tid & 15) * 4;
+    const int64_t e = e0 + n;
+    double x[4];
+    double mx = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u;
+      x[u] = (e < NE && k < D3) ? Xg[e * D3 + k] : 0.0;
+      mx = fmax(mx, fabs(x[u]));
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off, kWave));
+    const int ex = exponent_of(mx);
+    if ((tid & 15) == 0) eX[n] = ex;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = ldexp(x[u], -ex);
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      const int d0 = next_digit(x[0]), d1 = next_digit(x[1]), d2 = next_digit(x[2]), d3 = next_digit(x[3]);
+      *reinterpret_cast<int*>(&Xs[p][n][k0]) = pack4(d0, d1, d2, d3);
+    }
+  }
+  __syncthreads();
+
+  // Forward: T_c(m, n) = sum_k K_c(m, k) X_n(k), rows m in [32w, 32w + 32).
+  v4d t_acc[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) t_acc[c][i] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v4i kf[S][2];
+#pragma unroll
+    for (int p = 0; p < S; ++p)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        kf[p][i] = *reinterpret_cast<const v4i*>(Kf + ((size_t(p) * 3 + c) * kQP + wave * 32 + i * 16 + fr) * kDP +
+                                                 fk);
+#pragma unroll
+    for (int t = S - 1; t >= 0; --t) {
+      v4i acc[2] = {v4i{0, 0, 0, 0}, v4i{0, 0, 0, 0}};
+#pragma unroll
+      for (int p = 0; p <= t; ++p) {
+        asm volatile("" ::: "memory");
+        const v4i bf = *reinterpret_cast<const v4i*>(&Xs[t - p][fr][fk]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(kf[p][i], bf, acc[i], 0, 0, 0);
+      }
+      const double w = ldexp(1.0, -7 * (t + 2));
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) t_acc[c][i][r] += double(acc[i][r]) * w;
+    }
+  }
+
+  // Flux U_c = sum_c' O_cc' T_c' at the wavefront's points, and each
+  // element's max |U|.  Accumulator element r of lane l is row 4*(l/16)+r,
+  // column (element) l%16.
+  const int n = fr;
+  const int64_t e = e0 + n;
+  double u[3][2][4];
+  {
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = wave * 32 + i * 16 + 4 * (lane >> 4) + r;
+        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+        if (m < Q3 && e < NE) {
+          const double g0 = ldexp(t_acc[0][i][r], ea[m] + eX[n]);
+          const double g1 = ldexp(t_acc[1][i][r], ea[kQP + m] + eX[n]);
+          const double g2 = ldexp(t_acc[2][i][r], ea[2 * kQP + m] + eX[n]);
+          const double* o = Dg + e * (NC * Q3) + m;
+          const double O11 = o[0], O12 = o[Q3], O13 = o[2 * Q3];
+          const double O21 = SYMM ? O12 : o[3 * Q3];
+          const double O22 = SYMM ? o[3 * Q3] : o[4 * Q3];
+          const double O23 = SYMM ? o[4 * Q3] : o[5 * Q3];
+          const double O31 = SYMM ? O13 : o[6 * Q3];
+          const double O32 = SYMM ? O23 : o[7 * Q3];
+          const double O33 = SYMM ? o[5 * Q3] : o[8 * Q3];
+          u0 = ((O11 * g0) + (O12 * g1)) + (O13 * g2);
+          u1 = ((O21 * g0) + (O22 * g1)) + (O23 * g2);
+          u2 = ((O31 * g0) + (O32 * g1)) + (O33 * g2);
+        }
+        u[0][i][r] = u0;
+        u[1][i][r] = u1;
+        u[2][i][r] = u2;
+        mx = fmax(mx, fmax(fabs(u0), fmax(fabs(u1), fabs(u2))));
+      }
+    mx = fmax(mx, __shfl_xor(mx, 16, kWave));
+    mx = fmax(mx, __shfl_xor(mx, 32, kWave));
+    if (lane < 16) umax[wave][n] = mx;
+  }
+  __syncthreads();  // every wavefront is done with X's slices and has its max
+
+  // U -> slices, one power-of-two scale per element.
+  const int eu = exponent_of(fmax(fmax(umax[0][n], umax[1][n]), fmax(umax[2][n], umax[3][n])));
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = ldexp(u[c][i][r], -eu);
+      const int m0 = wave * 32 + i * 16 + 4 * (lane >> 4);
+#pragma unroll
+      for (int p = 0; p < S; ++p) {
+        const int d0 = next_digit(v[0]), d1 = next_digit(v[1]), d2 = next_digit(v[2]), d3 = next_digit(v[3]);
+        *reinterpret_cast<int*>(&Us[p][n][c * kDUComp + m0]) = pack4(d0, d1, d2, d3);
+      }
+    }
+  __syncthreads();
+
+  // Backward: Y_n(j) += sum_c sum_m K_c(m, j) U_c,n(m), dof rows j in [16w, 16w + 16).
+  v4d y_acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v4i kb[S][kQP / 64];
+#pragma unroll
+    for (int p = 0; p < S; ++p)
+#pragma unroll
+      for (int ks = 0; ks < kQP / 64; ++ks)
+        kb[p][ks] = *reinterpret_cast<const v4i*>(Kb + ((size_t(p) * 3 + c) * kDP + wave * 16 + fr) * kQP +
+                                                  ks * 64 + fk);
+    v4d yc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = S - 1; t >= 0; --t) {
+      v4i acc = v4i{0, 0, 0, 0};
+#pragma unroll
+      for (int p = 0; p <= t; ++p) {
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int ks = 0; ks < kQP / 64; ++ks) {
+          const v4i bf = *reinterpret_cast<const v4i*>(&Us[t - p][fr][c * kDUComp + ks * 64 + fk]);
+          acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(kb[p][ks], bf, acc, 0, 0, 0);
+        }
+      }
+      const double w = ldexp(1.0, -7 * (t + 2));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yc[r] += double(acc[r]) * w;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y_acc[r] += ldexp(yc[r], eb[c * kDP + wave * 16 + 4 * (lane >> 4) + r]);
+  }
+  if (e < NE) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = wave * 16 + 4 * (lane >> 4) + r;
+      if (j < D3) Yg[e * D3 + j] += ldexp(y_acc[r], eu);
+    }
+  }
+}
+
+template <int D, int S, bool SYMM>
+int launch_diff_mixed(const double* Dq, const double* X, double* Y, int64_t NE, const DiffMixedLayout& l,
+                      char* base, hipStream_t s) {
+  const int64_t grid = ceil_div(NE, kDEB);
+  if (grid > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  diffusion3dpa_mixed_kernel<D, D + 1, S, SYMM><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(
+      reinterpret_cast<const signed char*>(base + l.kf_off), reinterpret_cast<const signed char*>(base + l.kb_off),
+      reinterpret_cast<const int*>(base + l.ea_off), reinterpret_cast<const int*>(base + l.eb_off), Dq, X, Y, NE);
+  return launch_status();
+}
+
+template <int S, bool SYMM>
+int run_diff_order(const double* Dq, const double* X, double* Y, int64_t NE, int d1d, const DiffMixedLayout& l,
+                   char* base, hipStream_t s) {
+  switch (d1d) {
+    case 1: return launch_diff_mixed<1, S, SYMM>(Dq, X, Y, NE, l, base, s);
+    case 2: return launch_diff_mixed<2, S, SYMM>(Dq, X, Y, NE, l, base, s);
+    case 3: return launch_diff_mixed<3, S, SYMM>(Dq, X, Y, NE, l, base, s);
+    case 4: return launch_diff_mixed<4, S, SYMM>(Dq, X, Y, NE, l, base, s);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+template <int S>
+int run_diff_slices(const double* Bg, const double* Gg, const double* Dq, const double* X, double* Y, int64_t NE,
+                    int d1d, int symmetric, const DiffMixedLayout& l, char* base, hipStream_t s) {
+  diff_kron_slices_kernel<S><<<3, 256, 0, s>>>(Bg, Gg, d1d, d1d + 1, reinterpret_cast<signed char*>(base + l.kf_off),
+                                               reinterpret_cast<signed char*>(base + l.kb_off),
+                                               reinterpret_cast<int*>(base + l.ea_off),
+                                               reinterpret_cast<int*>(base + l.eb_off));
+  if (const int rc = launch_status(); rc != RPX_SUCCESS) return rc;
+  return symmetric ? run_diff_order<S, true>(Dq, X, Y, NE, d1d, l, base, s)
+                   : run_diff_order<S, false>(Dq, X, Y, NE, d1d, l, base, s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -438,6 +755,33 @@ int rpx_mass3dpa_mixed_f64(const double* B, const double* Bt, const double* D, c
     case 6: return run_slices<6>(B, Bt, D, X, Y, NE, d1d, q1d, l, base, st);
     case 7: return run_slices<7>(B, Bt, D, X, Y, NE, d1d, q1d, l, base, st);
     case 8: return run_slices<8>(B, Bt, D, X, Y, NE, d1d, q1d, l, base, st);
+    default: return RPX_ERR_UNSUPPORTED;
+  }
+}
+
+size_t rpx_diffusion3dpa_mixed_scratch_bytes(int d1d, int q1d, int nslices) {
+  const int s = nslices == 0 ? 7 : nslices;
+  if (d1d < 1 || d1d > 4 || q1d != d1d + 1 || s < 5 || s > kMaxSlices) return 0;
+  return diff_layout(s).total;
+}
+
+int rpx_diffusion3dpa_mixed_f64(const double* Basis, const double* dBasis, const double* D, const double* X,
+                                double* Y, int64_t NE, int d1d, int q1d, int symmetric, int nslices, void* scratch,
+                                size_t scratch_bytes, void* stream) {
+  if (NE < 0 || nslices < 0 || nslices > kMaxSlices) return RPX_ERR_INVALID_ARGUMENT;
+  if (NE == 0) return RPX_SUCCESS;
+  if (!Basis || !dBasis || !D || !X || !Y || !scratch) return RPX_ERR_INVALID_ARGUMENT;
+  if (d1d < 1 || d1d > 4 || q1d != d1d + 1) return RPX_ERR_UNSUPPORTED;
+  const int s = nslices == 0 ? 7 : nslices;
+  const DiffMixedLayout l = diff_layout(s);
+  if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
+  char* base = reinterpret_cast<char*>(scratch);
+  hipStream_t st = as_stream(stream);
+  switch (s) {
+    case 5: return run_diff_slices<5>(Basis, dBasis, D, X, Y, NE, d1d, symmetric, l, base, st);
+    case 6: return run_diff_slices<6>(Basis, dBasis, D, X, Y, NE, d1d, symmetric, l, base, st);
+    case 7: return run_diff_slices<7>(Basis, dBasis, D, X, Y, NE, d1d, symmetric, l, base, st);
+    case 8: return run_diff_slices<8>(Basis, dBasis, D, X, Y, NE, d1d, symmetric, l, base, st);
     default: return RPX_ERR_UNSUPPORTED;
   }
 }

@@ -10,7 +10,8 @@
 //       x = sum_{p<s} d_p 2^{-7(p+1)} + r,  |d_p| <= 127,  |r| < 2^{-7s},
 //     by exact FP64 operations (scale by 128, truncate, subtract);
 //   * S_t = sum_{p+q=t} sum_k dA_p[i][k] dB_q[k][j] is an exact int32 sum for
-//     every t < s (|S_t| <= 7*127^2*nk < 2^31 for nk <= 19000);
+//     every t < s: level t has at most s digit pairs, so |S_t| <= s*127^2*nk,
+//     below 2^31 for nk <= (2^31-1)/(s*127^2) (19020 at s = 7, 16643 at 8);
 //   * C[i][j] = alpha * 2^(e_i+f_j) * sum_{t<s} S_t 2^{-7(t+2)}, summed in
 //     FP64 from the smallest term up.
 // The dropped digit products are below (s+3) 2^{-7s} relative to
@@ -33,7 +34,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kMaxSlices = 8;
 constexpr int kTM = 256, kTN = 256, kTK = 128;  // block tile, K bytes per stage
 constexpr int kChunks = kTK / 16;               // 16-byte chunks per tile row and stage
-constexpr int64_t kMaxK = 19000;                // keeps every S_t inside int32
+// Largest nk for which every S_t stays inside int32 with s slices.
+constexpr int64_t max_k(int s) { return int64_t(INT32_MAX) / (int64_t(s) * 127 * 127); }
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -376,8 +378,8 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
   if (ni < 0 || nj < 0 || nk < 0 || nslices < 0 || nslices > kMaxSlices) return RPX_ERR_INVALID_ARGUMENT;
   if (ni == 0 || nj == 0) return RPX_SUCCESS;
   if (!C || !scratch || (nk > 0 && (!A || !B))) return RPX_ERR_INVALID_ARGUMENT;
-  if (nk > kMaxK) return RPX_ERR_UNSUPPORTED;
   const int s = nslices == 0 ? 7 : nslices;
+  if (nk > max_k(s)) return RPX_ERR_UNSUPPORTED;
   const MixedLayout l = mixed_layout(ni, nj, nk, s);
   if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
   if (l.nip > INT32_MAX || (l.nip / kTM) * (l.njp / kTN) > INT32_MAX || ceil_div(ni * nj, 256) > INT32_MAX)

@@ -22,8 +22,9 @@
 //    atomic per tile serialises on its single address -- measured 5.5 ms
 //    against 0.8 ms for 2^28 doubles on MI355X.)
 //  * the look-back is done by one wavefront over windows of 64
-//    predecessors; flags carry an epoch so the scratch needs no clearing
-//    between calls.  The last tile advances the epoch after its look-back:
+//    predecessors; flags carry a tag (header epoch + 1) so the scratch needs
+//    no clearing between calls.  The last tile advances the epoch after its
+//    look-back:
 //    by then every tile has published a flag, so every tile has read the
 //    epoch;
 //  * every spin is bounded; a wait that expires records RPX_ERR_WAIT_EXPIRED
@@ -196,7 +197,10 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
   }
   if (lane == 0) wave_tot[wave] = carry;
   __syncthreads();
-  const unsigned long long E = SEG ? 0ull : s_epoch;
+  // Flag tag: 0 for the segmented form (its flags are cleared before every
+  // launch), epoch + 1 for the single-array form, so a single-array scan
+  // never accepts a flag a segmented call left in a shared scratch.
+  const unsigned long long E = SEG ? 0ull : s_epoch + 1;
   double wave_off = 0.0, tile_agg = 0.0;
 #pragma unroll
   for (int w = 0; w < BS / kWave; ++w) {

@@ -5,6 +5,8 @@
 #include <stdexcept>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "rajaperf_mi355x.h"
 
 namespace rajaperf {
@@ -32,6 +34,26 @@ void deviceFree(void* p) { check(rpx_device_free(p), "rpx_device_free"); }
 void copyToDevice(void* dst, const void* src, Size_type bytes) { check(rpx_memcpy_h2d(dst, src, bytes), "h2d"); }
 void copyToHost(void* dst, const void* src, Size_type bytes) { check(rpx_memcpy_d2h(dst, src, bytes), "d2h"); }
 void deviceSynchronize() { check(rpx_device_synchronize(), "rpx_device_synchronize"); }
+
+void* captureStream() {
+  static void* s = nullptr;  // created once, kept until the process ends
+  if (!s) check(rpx_stream_create(&s), "rpx_stream_create");
+  return s;
+}
+void beginCapture(void* stream) { check(rpx_graph_begin_capture(stream), "rpx_graph_begin_capture"); }
+void* endCapture(void* stream) {
+  void* g = nullptr;
+  check(rpx_graph_end_capture(stream, &g), "rpx_graph_end_capture");
+  return g;
+}
+void abandonCapture(void* stream) {
+  void* g = nullptr;
+  if (rpx_graph_end_capture(stream, &g) == RPX_SUCCESS) rpx_graph_destroy(g);
+}
+void launchGraph(void* graph, void* stream) { check(rpx_graph_launch(graph, stream), "rpx_graph_launch"); }
+void destroyGraph(void* graph) { check(rpx_graph_destroy(graph), "rpx_graph_destroy"); }
+void rangePush(const std::string& name) { roctxRangePushA(name.c_str()); }
+void rangePop() { roctxRangePop(); }
 
 void initDataDevice(Real_ptr d, Size_type len) {
   const Real_type factor = (data_init_count % 2 ? 0.1 : 0.2);
@@ -196,10 +218,17 @@ long double calcChecksumHost(const Real_type* h, Size_type len, Real_type scale_
   return tchk;
 }
 
-long double calcChecksumDevice(const Real_type* d, Size_type len, Real_type scale_factor) {
+long double calcChecksumDevice(const Real_type* d, Size_type len, Real_type scale_factor, double* iweighted) {
   std::vector<Real_type> h(len);
   if (len) copyToHost(h.data(), d, len * sizeof(Real_type));
+  if (iweighted) *iweighted += iWeightedSumHost(h.data(), len);
   return calcChecksumHost(h.data(), len, scale_factor);
+}
+
+double iWeightedSumHost(const Real_type* h, Size_type len) {
+  double s = 0.0;
+  for (Size_type i = 0; i < len; ++i) s += static_cast<double>(i + 1) * h[i];
+  return s;
 }
 
 }  // namespace detail

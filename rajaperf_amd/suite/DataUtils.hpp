@@ -11,6 +11,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "RPTypes.hpp"
@@ -29,6 +30,18 @@ void copyToDevice(void* dst, const void* src_host, Size_type bytes);
 void copyToHost(void* dst_host, const void* src, Size_type bytes);
 void deviceSynchronize();
 
+// HIP graph capture of rep loops (KernelBase::timedReps): one created stream
+// per process, capture, replay and teardown; errors throw.
+void* captureStream();
+void beginCapture(void* stream);
+void* endCapture(void* stream);
+void abandonCapture(void* stream);
+void launchGraph(void* graph, void* stream);
+void destroyGraph(void* graph);
+// roctx ranges around timed rep loops (no-ops when no profiler listens).
+void rangePush(const std::string& name);
+void rangePop();
+
 // Initialisers writing device arrays; each advances the init counter once.
 void initDataDevice(Real_ptr d, Size_type len);                   // DataUtils.cpp:504-513
 void initDataConstDevice(Real_ptr d, Size_type len, Real_type v);  // :518-528
@@ -44,7 +57,14 @@ void initData(Real_type& d);                                        // :589-595 
 void femBasis(int d1d, int q1d, std::vector<Real_type>& B, std::vector<Real_type>& G);
 
 // Checksums of device arrays (DataUtils.hpp:386-409, DataUtils.cpp:601-621).
-long double calcChecksumDevice(const Real_type* d, Size_type len, Real_type scale_factor);
+// The suite checksum of device data; with `iweighted`, also adds the
+// data's i-weighted sum to *iweighted.
+long double calcChecksumDevice(const Real_type* d, Size_type len, Real_type scale_factor,
+                               double* iweighted = nullptr);
+// sum_i (i+1)*h[i] in FP64, index order, no fused multiply-add: the second
+// per-kernel checksum (the "sum of i-weighted elements in FP64" of the
+// stream workload); weights start at 1 so a scalar result is itself.
+double iWeightedSumHost(const Real_type* h, Size_type len);
 long double calcChecksumHost(const Real_type* h, Size_type len, Real_type scale_factor);
 
 }  // namespace detail

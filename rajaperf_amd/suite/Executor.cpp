@@ -144,7 +144,7 @@ void Executor::loadProgress() {
   std::string line;
   while (std::getline(f, line)) {
     std::string kname, vname, tname;
-    double hi, lo, tot, mn, mx, passes, reps, size;
+    double hi, lo, tot, mn, mx, passes, reps, size, iw = 0.0;
     if (!json_string(line, "kernel", kname) || !json_string(line, "variant", vname) ||
         !json_string(line, "tuning", tname) || !json_number(line, "checksum_hi", hi) ||
         !json_number(line, "checksum_lo", lo) || !json_number(line, "time_total_s", tot) ||
@@ -152,6 +152,7 @@ void Executor::loadProgress() {
         !json_number(line, "passes", passes) || !json_number(line, "reps", reps) ||
         !json_number(line, "problem_size", size))
       continue;  // not a complete record (e.g. a line cut short by a crash)
+    json_number(line, "iweighted", iw);
     for (KernelBase* k : kernels) {
       if (k->getName() != kname) continue;
       for (VariantID vid : variant_ids) {
@@ -160,7 +161,8 @@ void Executor::loadProgress() {
         if (t == KernelBase::getUnknownTuningIdx() || !tuningSelected(tname)) continue;
         if (double(k->getActualProblemSize()) != size || double(k->getRunReps()) != reps) continue;
         if (int(passes) < run_params.getNumPasses()) continue;  // an incomplete record is rerun
-        k->restoreRun(vid, t, static_cast<long double>(hi) + static_cast<long double>(lo), tot, mn, mx, int(passes));
+        k->restoreRun(vid, t, static_cast<long double>(hi) + static_cast<long double>(lo), iw, tot, mn, mx,
+                      int(passes));
         restored.insert(run_key(kname, vname, tname));
       }
     }
@@ -174,7 +176,8 @@ void Executor::appendProgress(const KernelBase* k, VariantID vid, size_t t) cons
   split_ld(k->getChecksum(vid, t), hi, lo);
   f << std::setprecision(17) << "{\"kernel\": \"" << json_escape(k->getName()) << "\", \"variant\": \""
     << json_escape(getVariantName(vid)) << "\", \"tuning\": \"" << json_escape(k->getVariantTuningName(vid, t))
-    << "\", \"checksum_hi\": " << hi << ", \"checksum_lo\": " << lo << ", \"time_total_s\": " << k->getTotTime(vid, t)
+    << "\", \"checksum_hi\": " << hi << ", \"checksum_lo\": " << lo << ", \"iweighted\": " << k->getIWeighted(vid, t)
+    << ", \"time_total_s\": " << k->getTotTime(vid, t)
     << ", \"time_min_s\": " << k->getMinTime(vid, t) << ", \"time_max_s\": " << k->getMaxTime(vid, t)
     << ", \"passes\": " << k->getNumExec(vid, t) << ", \"reps\": " << k->getRunReps()
     << ", \"problem_size\": " << k->getActualProblemSize() << "}" << std::endl;
@@ -259,28 +262,36 @@ void Executor::outputRunData() {  // Executor.cpp:825-883
   const std::string dir = run_params.getOutputDirName();
   make_dir(dir);
   const std::string pre = dir + "/" + run_params.getOutputFilePrefix();
-  writeTimingCSV(pre + "-timing-Average.csv");
+  for (const std::string& c : run_params.getNpassesCombiners()) writeTimingCSV(pre + "-timing-" + c + ".csv", c);
   writeChecksumReport(pre + "-checksum.txt");
   writeKernelInfoCSV(pre + "-kernels.csv");
   writeRooflineCSV(pre + "-roofline.csv");
   if (!run_params.getJsonFile().empty()) writeJson(run_params.getJsonFile());
 }
 
-void Executor::writeTimingCSV(const std::string& path) const {
+// One timing report per --npasses-combiners entry (Executor.cpp:852-854,
+// 1515-1600): the pass times of each run combined as their mean
+// (total / passes), minimum or maximum, in seconds per pass, then per rep
+// with the rates that per-rep time gives.
+void Executor::writeTimingCSV(const std::string& path, const std::string& combiner) const {
   std::ofstream f(path);
-  f << "Mean Runtime Report (sec.)\nKernel,Variant,Tuning,Time/rep,GB/s,GFLOP/s,Frac HBM (8 TB/s)\n";
+  const char* title = combiner == "Minimum" ? "Min" : combiner == "Maximum" ? "Max" : "Mean";
+  f << title << " Runtime Report (sec.)\nKernel,Variant,Tuning,Time/pass,Time/rep,GB/s,GFLOP/s,Frac HBM (8 TB/s)\n";
   for (const KernelBase* k : kernels)
     for (VariantID vid : variant_ids)
       for (size_t t = 0; t < k->getNumVariantTunings(vid); ++t) {
         f << k->getName() << "," << getVariantName(vid) << "," << k->getVariantTuningName(vid, t) << ",";
         if (!k->wasVariantTuningRun(vid, t)) {
-          f << "Not run,,,\n";  // unexecuted tunings are marked, not reported as 0 s
+          f << "Not run,,,,\n";  // unexecuted tunings are marked, not reported as 0 s
           continue;
         }
-        const double per_rep = k->getTotTime(vid, t) / k->getNumExec(vid, t) / double(std::max<Index_type>(1, k->getRunReps()));
+        const double per_pass = combiner == "Minimum"   ? k->getMinTime(vid, t)
+                                : combiner == "Maximum" ? k->getMaxTime(vid, t)
+                                                        : k->getTotTime(vid, t) / k->getNumExec(vid, t);
+        const double per_rep = per_pass / double(std::max<Index_type>(1, k->getRunReps()));
         const double gbs = k->getBytesPerRep() / per_rep / 1e9;
-        f << std::setprecision(8) << per_rep << "," << gbs << "," << k->getFLOPsPerRep() / per_rep / 1e9 << ","
-          << gbs / 8000.0 << "\n";
+        f << std::setprecision(8) << per_pass << "," << per_rep << "," << gbs << ","
+          << k->getFLOPsPerRep() / per_rep / 1e9 << "," << gbs / 8000.0 << "\n";
       }
 }
 
@@ -303,7 +314,8 @@ void Executor::writeChecksumReport(const std::string& path) const {  // Executor
           ref = cs;
           have_ref = true;
         }
-        f << std::setprecision(20) << cs << "  diff " << std::setprecision(6) << (cs - ref) << "\n";
+        f << std::setprecision(20) << cs << "  diff " << std::setprecision(6) << (cs - ref)
+          << "  i-weighted FP64 sum " << std::setprecision(17) << k->getIWeighted(vid, t) << "\n";
       }
   }
 }
@@ -413,8 +425,10 @@ void Executor::writeJson(const std::string& path) const {
         split_ld(k->getChecksum(vid, t), hi, lo);
         f << (first ? "" : ", ") << std::setprecision(17) << "{\"kernel\": \"" << k->getName()
           << "\", \"variant\": \"" << getVariantName(vid) << "\", \"tuning\": \"" << k->getVariantTuningName(vid, t)
-          << "\", \"checksum_hi\": " << hi << ", \"checksum_lo\": " << lo << ", \"time_total_s\": "
-          << k->getTotTime(vid, t) << ", \"time_min_s\": " << k->getMinTime(vid, t) << ", \"passes\": "
+          << "\", \"checksum_hi\": " << hi << ", \"checksum_lo\": " << lo << ", \"iweighted\": "
+          << k->getIWeighted(vid, t) << ", \"time_total_s\": "
+          << k->getTotTime(vid, t) << ", \"time_min_s\": " << k->getMinTime(vid, t) << ", \"time_max_s\": "
+          << k->getMaxTime(vid, t) << ", \"passes\": "
           << k->getNumExec(vid, t) << ", \"reps\": " << k->getRunReps() << ", \"problem_size\": "
           << k->getActualProblemSize() << ", \"bytes_per_rep\": " << k->getBytesPerRep()
           << ", \"flops_per_rep\": " << k->getFLOPsPerRep() << "}";

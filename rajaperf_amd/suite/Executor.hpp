@@ -31,7 +31,7 @@ class Executor {
 
  private:
   void runKernel(KernelBase* kern);
-  void writeTimingCSV(const std::string& path) const;
+  void writeTimingCSV(const std::string& path, const std::string& combiner) const;
   void writeChecksumReport(const std::string& path) const;
   void writeKernelInfoCSV(const std::string& path) const;
   void writeJson(const std::string& path) const;

@@ -20,9 +20,16 @@ Index_type KernelBase::getRunReps() const {  // KernelBase.cpp:127-136
 
 void KernelBase::setVariantDefined(VariantID vid) {  // KernelBase.cpp:138-232
   if (!isVariantAvailable(vid)) return;
-  if (vid == MI355X_Native) setMI355XTuningDefinitions(vid);
+  if (vid == MI355X_Native) {
+    setMI355XTuningDefinitions(vid);
+    num_base_tunings = getNumVariantTunings(vid);
+    if (run_params.getGraphMode() && graphCapturable())
+      for (size_t t = 0; t < num_base_tunings; ++t)
+        addVariantTuningName(vid, getVariantTuningName(vid, t) + "-graph");
+  }
   const size_t n = getNumVariantTunings(vid);
   checksum[vid].assign(n, 0.0L);
+  iweighted[vid].assign(n, 0.0);
   num_exec[vid].assign(n, 0);
   min_time[vid].assign(n, std::numeric_limits<double>::max());
   max_time[vid].assign(n, -std::numeric_limits<double>::max());
@@ -43,15 +50,18 @@ void KernelBase::execute(VariantID vid, size_t tune_idx) {  // KernelBase.cpp:35
   detail::resetDataInitCount();
   this->setUp(vid, tune_idx);
   this->runKernel(vid, tune_idx);
+  iw_pass = 0.0;
   this->updateChecksum(vid, tune_idx);
+  iweighted[vid].at(tune_idx) += iw_pass;
   this->tearDown(vid, tune_idx);
   running_variant = NumVariants;
   running_tuning = getUnknownTuningIdx();
 }
 
-void KernelBase::restoreRun(VariantID vid, size_t t, Checksum_type cs, double tot, double mn, double mx,
+void KernelBase::restoreRun(VariantID vid, size_t t, Checksum_type cs, double iw, double tot, double mn, double mx,
                             int nexec) {
   checksum[vid].at(t) = cs;
+  iweighted[vid].at(t) = iw;
   tot_time[vid].at(t) = tot;
   min_time[vid].at(t) = mn;
   max_time[vid].at(t) = mx;
@@ -68,7 +78,19 @@ void KernelBase::recordExecTime() {  // KernelBase.cpp:379-389
 void KernelBase::runKernel(VariantID vid, size_t tune_idx) {  // KernelBase.cpp:391-489
   if (!hasVariantDefined(vid)) return;
   switch (vid) {
-    case MI355X_Native: runMI355XVariant(vid, tune_idx); break;
+    case MI355X_Native: {
+      // "-graph" tunings run their base tuning with the rep loop captured
+      graph_active = tune_idx >= num_base_tunings;
+      const size_t base = graph_active ? tune_idx - num_base_tunings : tune_idx;
+      try {
+        runMI355XVariant(vid, base);
+      } catch (...) {
+        graph_active = false;
+        throw;
+      }
+      graph_active = false;
+      break;
+    }
     default: getCout() << "\n  " << name << " : variant " << getVariantName(vid) << " is not built here\n"; break;
   }
 }

@@ -7,7 +7,8 @@
 // (KernelBase.cpp:359-377); runKernel dispatches on the VariantID to
 // runMI355XVariant (the counterpart of runHipVariant); startTimer/stopTimer
 // bracket the timed rep loop with a device synchronisation
-// (KernelBase.hpp:271-294, 423-442).  Per-kernel data lives in HBM.
+// (KernelBase.hpp:271-294, 423-442).  Per-kernel data lives in HBM.  The
+// rep loops go through timedReps, which can also replay them as HIP graphs.
 #pragma once
 
 #include <chrono>
@@ -50,6 +51,10 @@ class KernelBase {
 
   // Tunings of the new variant; the default is the library's tuned choice.
   virtual void setMI355XTuningDefinitions(VariantID vid) { addVariantTuningName(vid, getDefaultTuningName()); }
+  // Whether the rep loop can be captured in a HIP graph (under --graph each
+  // tuning then also runs as "<tuning>-graph").  Kernels whose calls upload
+  // per-call host tables say no: a graph would replay the first call's copy.
+  virtual bool graphCapturable() const { return true; }
 
   Index_type getDefaultProblemSize() const { return default_prob_size; }
   Index_type getActualProblemSize() const { return actual_prob_size; }
@@ -82,10 +87,53 @@ class KernelBase {
   double getTotTime(VariantID vid, size_t t) const { return tot_time[vid].at(t); }
   int getNumExec(VariantID vid, size_t t) const { return num_exec[vid].at(t); }
   Checksum_type getChecksum(VariantID vid, size_t t) const { return checksum[vid].at(t); }
+  // The i-weighted FP64 sums of the checksummed arrays, over passes.
+  double getIWeighted(VariantID vid, size_t t) const { return iweighted[vid].at(t); }
 
   void execute(VariantID vid, size_t tune_idx);
   // A run recorded by an earlier invocation (restored under --resume).
-  void restoreRun(VariantID vid, size_t t, Checksum_type cs, double tot, double mn, double mx, int nexec);
+  void restoreRun(VariantID vid, size_t t, Checksum_type cs, double iw, double tot, double mn, double mx,
+                  int nexec);
+
+  // The stream of the timed rep loop's device work: the null stream, or the
+  // capture stream in a "-graph" tuning.
+  void* stream() const { return graph_active ? detail::captureStream() : nullptr; }
+
+  // The timed rep loop: startTimer, `reps` calls of body(irep), stopTimer
+  // (KernelBase.hpp:423-442 around the loops of every run<Backend>Variant).
+  // In a "-graph" tuning all reps are captured into one HIP graph before the
+  // timer starts (so capture and instantiation are not timed) and the timer
+  // brackets one replay.  Either way the loop is a named roctx range,
+  // "<kernel>.<variant>.<tuning>", for profilers.
+  template <typename Body>
+  void timedReps(Index_type reps, Body&& body) {
+    const std::string range = name + "." + getVariantName(running_variant) + "." +
+                              getVariantTuningName(running_variant, running_tuning);
+    if (!graph_active) {
+      detail::rangePush(range);
+      startTimer();
+      for (RepIndex_type irep = 0; irep < reps; ++irep) body(irep);
+      stopTimer();
+      detail::rangePop();
+      return;
+    }
+    void* s = detail::captureStream();
+    void* graph = nullptr;
+    detail::beginCapture(s);
+    try {
+      for (RepIndex_type irep = 0; irep < reps; ++irep) body(irep);
+    } catch (...) {
+      detail::abandonCapture(s);
+      throw;
+    }
+    graph = detail::endCapture(s);
+    detail::rangePush(range);
+    startTimer();
+    detail::launchGraph(graph, s);
+    stopTimer();
+    detail::rangePop();
+    detail::destroyGraph(graph);
+  }
 
   void synchronize() {
     if (running_variant == MI355X_Native) detail::deviceSynchronize();
@@ -120,8 +168,10 @@ class KernelBase {
   bool seededInputs() const { return run_params.getInputMode() == RunParams::SeededInputs; }
 
   long double calcChecksum(const Real_type* dptr, Size_type len, Real_type scale = 1.0) {
-    return detail::calcChecksumDevice(dptr, len, scale);
+    return detail::calcChecksumDevice(dptr, len, scale, &iw_pass);
   }
+  // A host-side result's contribution to the i-weighted sum (scalars).
+  void addIWeightedHost(const Real_type* h, Size_type len) { iw_pass += detail::iWeightedSumHost(h, len); }
 
   // Tuning for the variant: explicit values from --gpu_block_size /
   // --items_per_thread apply to the default tuning.
@@ -136,6 +186,8 @@ class KernelBase {
  protected:
   const RunParams& run_params;
   std::vector<Checksum_type> checksum[NumVariants];
+  std::vector<double> iweighted[NumVariants];
+  double iw_pass = 0.0;  // this pass's i-weighted sum, added after updateChecksum
   Checksum_type checksum_scale_factor = 1.0;
 
  private:
@@ -149,6 +201,8 @@ class KernelBase {
   Index_type FLOPs_per_rep = -1;
   Complexity complexity = NumComplexities;
   std::vector<std::string> variant_tuning_names[NumVariants];
+  size_t num_base_tunings = 0;  // tunings before the "-graph" copies
+  bool graph_active = false;
   VariantID running_variant = NumVariants;
   size_t running_tuning = getUnknownTuningIdx();
   std::vector<int> num_exec[NumVariants];

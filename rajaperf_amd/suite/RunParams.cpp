@@ -128,6 +128,12 @@ void RunParams::parse(int argc, const char** argv) {
       auto l = take_list(argc, argv, i);
       need(!l.empty());
       exclude_tuning_input.insert(exclude_tuning_input.end(), l.begin(), l.end());
+    } else if (opt == "--npasses-combiners") {  // RunParams.cpp:386-399
+      auto l = take_list(argc, argv, i);
+      need(!l.empty());
+      npasses_combiners.insert(npasses_combiners.end(), l.begin(), l.end());
+    } else if (opt == "--graph") {
+      graph_mode = true;
     } else if (opt == "--input-mode") {
       if (need(i + 1 < argc)) {
         const std::string m(argv[++i]);
@@ -150,6 +156,13 @@ void RunParams::parse(int argc, const char** argv) {
       input_state = BadInput;
     }
   }
+  // Combiners of the per-pass times (RunParams.cpp:1858-1878): Average by default.
+  if (npasses_combiners.empty()) npasses_combiners.push_back("Average");
+  for (const auto& c : npasses_combiners)
+    if (c != "Average" && c != "Minimum" && c != "Maximum") {
+      invalid_input.push_back(c);
+      input_state = BadInput;
+    }
   if (input_state == Undefined) input_state = PerfRun;
   if (input_state == InfoRequest) {
     if (print_kernels)
@@ -229,6 +242,8 @@ void RunParams::printHelpMessage(std::ostream& str) const {
       << "  --size <n> | --sizefact <f> target problem size, direct or as a factor\n"
       << "  --repfact <f>               scale the default rep counts\n"
       << "  --npasses <n>               passes over the suite\n"
+      << "  --npasses-combiners <Average|Minimum|Maximum...>  one timing CSV per combiner\n"
+      << "  --graph                     also run every tuning as a HIP graph of its rep loop (<tuning>-graph)\n"
       << "  --checkrun [reps]           run each kernel for <reps> reps (default 1)\n"
       << "  --gpu_block_size <n>        workgroup size for the default tuning\n"
       << "  --items_per_thread <n>      items per work-item for the default tuning\n"

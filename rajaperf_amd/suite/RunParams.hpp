@@ -9,7 +9,8 @@
 // -h/--help.  Added for this variant: --input-mode {reference,seeded},
 // --seed, --tsteps (JACOBI_2D), --nk (GEMM), --d1d/--q1d (FEM orders),
 // --json <file> (machine-readable results), --tuning-cache <file> (the
-// autotuner's decisions for the "auto" tunings).
+// autotuner's decisions for the "auto" tunings), --graph (graph-replayed rep
+// loops).  --npasses-combiners is the suite's.
 #pragma once
 
 #include <cstdint>
@@ -35,6 +36,12 @@ class RunParams {
   std::int64_t getSize() const { return size; }
   double getRepFactor() const { return rep_fact; }
   int getNumPasses() const { return npasses; }
+  // How the per-pass times of a run are combined in the timing reports:
+  // "Average", "Minimum", "Maximum" (RunParams.hpp CombinerOpt).
+  const std::vector<std::string>& getNpassesCombiners() const { return npasses_combiners; }
+  // --graph: every MI355X tuning is also run as "<tuning>-graph", its timed
+  // rep loop captured in a HIP graph outside the timer and replayed inside.
+  bool getGraphMode() const { return graph_mode; }
   int getCheckRunReps() const { return checkrun_reps; }
   bool showProgress() const { return show_progress; }
   bool getDisableWarmup() const { return disable_warmup; }
@@ -79,6 +86,8 @@ class RunParams {
   std::int64_t size = 0;
   double rep_fact = 1.0;
   int npasses = 1;
+  std::vector<std::string> npasses_combiners;
+  bool graph_mode = false;
   int checkrun_reps = 1;
   bool show_progress = false;
   bool disable_warmup = false;

@@ -31,6 +31,7 @@ void REDUCE_SUM::setUp(VariantID, size_t) {
 }
 void REDUCE_SUM::updateChecksum(VariantID vid, size_t t) {
   checksum[vid].at(t) += detail::calcChecksumHost(&m_sum, 1, 1.0);  // REDUCE_SUM.cpp:73-76
+  addIWeightedHost(&m_sum, 1);
 }
 void REDUCE_SUM::tearDown(VariantID, size_t) { deallocData(m_x); }
 void REDUCE_SUM::runMI355XVariant(VariantID, size_t tune_idx) {
@@ -44,10 +45,9 @@ void REDUCE_SUM::runMI355XVariant(VariantID, size_t tune_idx) {
     rpxCheck(rpx_reduce_sum_f64(m_x, n, d_sum, scratch, sb, &t, nullptr), "REDUCE_SUM");
   });
   // m_sum = sum every rep (REDUCE_SUM-Seq.cpp:38-45): the last rep's value.
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_reduce_sum_f64(m_x, n, d_sum, scratch, sb, &tune, nullptr), "REDUCE_SUM");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_reduce_sum_f64(m_x, n, d_sum, scratch, sb, &tune, stream()), "REDUCE_SUM");
+  });
   if (reps > 0) detail::copyToHost(&m_sum, d_sum, sizeof(Real_type));
   detail::deviceFree(scratch);
 }
@@ -106,10 +106,9 @@ void SCAN::runMI355XVariant(VariantID, size_t tune_idx) {
                          rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, &t, nullptr), "SCAN");
                        })
                .tuning;
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, &tune, nullptr), "SCAN");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_scan_exclusive_f64(m_x, m_y, n, scratch, sb, &tune, stream()), "SCAN");
+  });
   rpxCheck(rpx_scan_scratch_status(scratch), "SCAN look-back");
   detail::deviceFree(scratch);
 }
@@ -141,10 +140,9 @@ void SORT::runMI355XVariant(VariantID, size_t) {
   const size_t sb = rpx_sort_segmented_scratch_bytes(seg, 1);
   void* scratch = sb ? detail::deviceAlloc(sb) : nullptr;
   // One segment per rep at x + n*irep (SORT.hpp:21-25).
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_sort_segmented_f64(m_x + n * irep, seg, 1, scratch, sb, nullptr), "SORT");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type irep) {
+    rpxCheck(rpx_sort_segmented_f64(m_x + n * irep, seg, 1, scratch, sb, stream()), "SORT");
+  });
   if (scratch) detail::deviceFree(scratch);
 }
 

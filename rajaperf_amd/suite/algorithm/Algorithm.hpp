@@ -37,6 +37,9 @@ class SCAN : public KernelBase {
 class SORT : public KernelBase {
  public:
   explicit SORT(const RunParams& params);
+  // Each call uploads its segment table from host staging memory, which a
+  // graph replay would not redo: no "-graph" tunings.
+  bool graphCapturable() const override { return false; }
   void setUp(VariantID vid, size_t tune_idx) override;
   void updateChecksum(VariantID vid, size_t tune_idx) override;
   void tearDown(VariantID vid, size_t tune_idx) override;

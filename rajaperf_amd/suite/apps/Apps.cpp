@@ -81,20 +81,18 @@ void MASS3DPA::runMI355XVariant(VariantID, size_t tune_idx) {
     const size_t bytes = rpx_mass3dpa_mixed_scratch_bytes(m_D1D, m_Q1D, 7);
     char* scratch = nullptr;
     allocData(scratch, Size_type(bytes));
-    startTimer();
-    for (RepIndex_type irep = 0; irep < reps; ++irep)
-      rpxCheck(rpx_mass3dpa_mixed_f64(m_B, m_Bt, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, 7, scratch, bytes, nullptr),
+    timedReps(reps, [&](RepIndex_type) {
+      rpxCheck(rpx_mass3dpa_mixed_f64(m_B, m_Bt, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, 7, scratch, bytes, stream()),
                "MASS3DPA mixed");
-    stopTimer();
+    });
     deallocData(scratch);
     return;
   }
   rpx_tuning tune = defaultTuning();
   if (tune_idx == 1) tune.flags |= RPX_TUNE_NAIVE;
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_mass3dpa_f64(m_B, m_Bt, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, &tune, nullptr), "MASS3DPA");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_mass3dpa_f64(m_B, m_Bt, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, &tune, stream()), "MASS3DPA");
+  });
 }
 
 // ---- DIFFUSION3DPA --------------------------------------------------------------
@@ -151,14 +149,30 @@ void DIFFUSION3DPA::tearDown(VariantID, size_t) {
   deallocData(m_X);
   deallocData(m_Y);
 }
-void DIFFUSION3DPA::runMI355XVariant(VariantID, size_t) {
+void DIFFUSION3DPA::setMI355XTuningDefinitions(VariantID vid) {
+  addVariantTuningName(vid, "default");
+  // int8 MFMA on 7 Ozaki slices of the Kronecker form, where the order has it
+  if (rpx_diffusion3dpa_mixed_scratch_bytes(m_D1D, m_Q1D, 7) > 0) addVariantTuningName(vid, "mixed");
+}
+void DIFFUSION3DPA::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
+  if (tune_idx == 1) {  // the sliced bases' scratch is allocated outside the timed reps
+    const size_t bytes = rpx_diffusion3dpa_mixed_scratch_bytes(m_D1D, m_Q1D, 7);
+    char* scratch = nullptr;
+    allocData(scratch, Size_type(bytes));
+    timedReps(reps, [&](RepIndex_type) {
+      rpxCheck(rpx_diffusion3dpa_mixed_f64(m_B, m_G, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, 1, 7, scratch, bytes,
+                                           stream()),
+               "DIFFUSION3DPA mixed");
+    });
+    deallocData(scratch);
+    return;
+  }
   const rpx_tuning tune = defaultTuning();
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_diffusion3dpa_f64(m_B, m_G, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, 1, &tune, nullptr),
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_diffusion3dpa_f64(m_B, m_G, m_D, m_X, m_Y, m_NE, m_D1D, m_Q1D, 1, &tune, stream()),
              "DIFFUSION3DPA");
-  stopTimer();
+  });
 }
 
 }  // namespace apps

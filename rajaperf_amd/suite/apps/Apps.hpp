@@ -31,6 +31,7 @@ class DIFFUSION3DPA : public KernelBase {
   void updateChecksum(VariantID vid, size_t tune_idx) override;
   void tearDown(VariantID vid, size_t tune_idx) override;
   void runMI355XVariant(VariantID vid, size_t tune_idx) override;
+  void setMI355XTuningDefinitions(VariantID vid) override;
 
  private:
   static constexpr int SYM = 6;

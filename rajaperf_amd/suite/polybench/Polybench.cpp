@@ -58,20 +58,18 @@ void POLYBENCH_GEMM::runMI355XVariant(VariantID, size_t tune_idx) {
     const size_t bytes = rpx_gemm_mixed_scratch_bytes(m_ni, m_nj, m_nk, 7);
     char* scratch = nullptr;
     allocData(scratch, Size_type(bytes));
-    startTimer();
-    for (RepIndex_type irep = 0; irep < reps; ++irep)
-      rpxCheck(rpx_gemm_mixed_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, 7, scratch, bytes, nullptr),
+    timedReps(reps, [&](RepIndex_type) {
+      rpxCheck(rpx_gemm_mixed_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, 7, scratch, bytes, stream()),
                "GEMM mixed");
-    stopTimer();
+    });
     deallocData(scratch);
     return;
   }
   rpx_tuning tune = defaultTuning();
   if (tune_idx == 1) tune.flags |= RPX_TUNE_NAIVE;
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_gemm_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, &tune, nullptr), "GEMM");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_gemm_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, &tune, stream()), "GEMM");
+  });
 }
 
 // ---- JACOBI_2D -------------------------------------------------------------------
@@ -165,13 +163,11 @@ void POLYBENCH_JACOBI_2D::runMI355XVariant(VariantID, size_t tune_idx) {
     detail::deviceFree(tB);
   }
   const bool run_fused = fused || tune_idx == 5;
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep) {
+  timedReps(reps, [&](RepIndex_type) {
     if (run_fused)
-      rpxCheck(rpx_jacobi2d_fused_f64(m_A, m_B, m_N, m_tsteps, scratch, sb, &tune, nullptr), "JACOBI_2D");
-    else rpxCheck(rpx_jacobi2d_f64(m_A, m_B, m_N, m_tsteps, &tune, nullptr), "JACOBI_2D");
-  }
-  stopTimer();
+      rpxCheck(rpx_jacobi2d_fused_f64(m_A, m_B, m_N, m_tsteps, scratch, sb, &tune, stream()), "JACOBI_2D");
+    else rpxCheck(rpx_jacobi2d_f64(m_A, m_B, m_N, m_tsteps, &tune, stream()), "JACOBI_2D");
+  });
   if (scratch) detail::deviceFree(scratch);
 }
 

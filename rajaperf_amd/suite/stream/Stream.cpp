@@ -123,9 +123,9 @@ void COPY::runMI355XVariant(VariantID, size_t tune_idx) {
   const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
     rpxCheck(rpx_copy_f64(m_c, m_a, n, &t, nullptr), "COPY");
   });
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep) rpxCheck(rpx_copy_f64(m_c, m_a, n, &tune, nullptr), "COPY");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_copy_f64(m_c, m_a, n, &tune, stream()), "COPY");
+  });
 }
 
 // ---- MUL (src/stream/MUL.cpp:22-78) --------------------------------------------
@@ -159,10 +159,9 @@ void MUL::runMI355XVariant(VariantID, size_t tune_idx) {
   const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
     rpxCheck(rpx_mul_f64(m_b, m_c, m_alpha, n, &t, nullptr), "MUL");
   });
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_mul_f64(m_b, m_c, m_alpha, n, &tune, nullptr), "MUL");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_mul_f64(m_b, m_c, m_alpha, n, &tune, stream()), "MUL");
+  });
 }
 
 // ---- ADD (src/stream/ADD.cpp:22-78) --------------------------------------------
@@ -197,10 +196,9 @@ void ADD::runMI355XVariant(VariantID, size_t tune_idx) {
   const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
     rpxCheck(rpx_add_f64(m_c, m_a, m_b, n, &t, nullptr), "ADD");
   });
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_add_f64(m_c, m_a, m_b, n, &tune, nullptr), "ADD");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_add_f64(m_c, m_a, m_b, n, &tune, stream()), "ADD");
+  });
 }
 
 // ---- TRIAD (src/stream/TRIAD.cpp:22-84) ----------------------------------------
@@ -237,10 +235,9 @@ void TRIAD::runMI355XVariant(VariantID, size_t tune_idx) {
   const rpx_tuning tune = streamTuning(*this, tune_idx, [&](const rpx_tuning& t) {
     rpxCheck(rpx_triad_f64(m_a, m_b, m_c, m_alpha, n, &t, nullptr), "TRIAD");
   });
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_triad_f64(m_a, m_b, m_c, m_alpha, n, &tune, nullptr), "TRIAD");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_triad_f64(m_a, m_b, m_c, m_alpha, n, &tune, stream()), "TRIAD");
+  });
 }
 
 // ---- DOT (src/stream/DOT.cpp:22-81, DOT-Seq.cpp:34-51) -------------------------
@@ -263,7 +260,10 @@ void DOT::setUp(VariantID, size_t) {
   allocAndInitData(m_b, getActualProblemSize(), 0.0, 1.0);
   m_dot = 0.0;
 }
-void DOT::updateChecksum(VariantID vid, size_t t) { checksum[vid][t] += m_dot; }
+void DOT::updateChecksum(VariantID vid, size_t t) {
+  checksum[vid][t] += m_dot;
+  addIWeightedHost(&m_dot, 1);
+}
 void DOT::tearDown(VariantID, size_t) {
   deallocData(m_a);
   deallocData(m_b);
@@ -283,10 +283,9 @@ void DOT::runMI355XVariant(VariantID, size_t tune_idx) {
   });
   // m_dot += dot every rep (DOT-Seq.cpp:39-45), accumulated on the device in
   // rep order and read back once after the timed loop.
-  startTimer();
-  for (RepIndex_type irep = 0; irep < reps; ++irep)
-    rpxCheck(rpx_dot_f64(m_a, m_b, n, d_dot, 1, scratch, sb, &tune, nullptr), "DOT");
-  stopTimer();
+  timedReps(reps, [&](RepIndex_type) {
+    rpxCheck(rpx_dot_f64(m_a, m_b, n, d_dot, 1, scratch, sb, &tune, stream()), "DOT");
+  });
   Real_type dot = 0.0;
   detail::copyToHost(&dot, d_dot, sizeof(Real_type));
   m_dot += dot;

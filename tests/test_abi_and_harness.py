@@ -73,6 +73,15 @@ def test_harness_bad_input_runs_nothing():
     assert r.returncode == 1
 
 
+def test_harness_combiners_and_graph_flags_parse():
+    r = run_harness("--npasses-combiners", "Average", "Median")  # RunParams.cpp:1858-1878 accepts three
+    assert r.returncode == 1 and "Median" in r.stdout
+    r = run_harness("--dryrun", "-k", "Stream_TRIAD", "Algorithm_SORT", "--graph",
+                    "--npasses-combiners", "Average", "Minimum", "Maximum")
+    assert r.returncode == 0
+    assert "--graph" in run_harness("--help").stdout
+
+
 def test_harness_dryrun_shapes_match_reference_formulas():
     r = run_harness("--dryrun", "-k", "Polybench", "Apps", "--size", str(8190 * 8190))
     assert r.returncode == 0

@@ -66,6 +66,31 @@ def test_mixed_gemm_slices_trade_accuracy(rpx, rng):
         assert e <= 4 * (s + 3) * 2.0 ** (-7 * s) + 1e-15
 
 
+@pytest.mark.parametrize("s,nk_max", [(8, 16643), (7, 19020)])
+def test_mixed_gemm_nk_limit_all_digits_maximal(rpx, s, nk_max):
+    """The first seven base-128 digits of 1 - 2^-50 are 127 (the eighth is
+    64), so at s = 7 the top level of the int32 accumulation reaches
+    7 * 127^2 * nk: at the largest accepted nk it stays below 2^31 and the
+    result matches an 80-bit sum.  The limit is s * 127^2 * nk < 2^31 for
+    every s, and one more nk is refused."""
+    L = rpx.lib()
+    ni = nj = 16
+    v = 1.0 - 2.0 ** -50
+    assert s * 127 ** 2 * nk_max < 2 ** 31 <= s * 127 ** 2 * (nk_max + 1)
+    A = np.full(ni * nk_max, v)
+    B = np.full(nk_max * nj, v)
+    C = run_mixed(rpx, A, B, 1.0, ni, nj, nk_max, s)
+    ref = np.longdouble(nk_max) * np.longdouble(v) * np.longdouble(v)
+    assert np.all(np.abs(C - ref) <= (s + 3) * 2.0 ** (-7 * s) * nk_max + 4 * nk_max * 2.0 ** -53)
+    sb = L.rpx_gemm_mixed_scratch_bytes(ni, nj, nk_max + 1, s)
+    scr = rpx.DeviceBuffer(sb)
+    dA, dB, dC = rpx.DeviceBuffer(8 * ni * (nk_max + 1)), rpx.DeviceBuffer(8 * nj * (nk_max + 1)), \
+        rpx.DeviceBuffer(8 * ni * nj)
+    with pytest.raises(rpx.RpxError) as ei:
+        L.rpx_gemm_mixed_f64(dC.ptr, dA.ptr, dB.ptr, 1.0, 0.0, ni, nj, nk_max + 1, s, scr.ptr, sb, None)
+    assert ei.value.status == rpx.RPX_ERR_UNSUPPORTED
+
+
 def test_mixed_gemm_exact_on_small_integers(rpx, rng):
     ni, nj, nk = 64, 48, 96
     A = rng.integers(-8, 8, ni * nk).astype(np.float64)
@@ -142,3 +167,107 @@ def test_mass3dpa_mixed_rejects_orders_and_small_scratch(rpx):
         L.rpx_mass3dpa_mixed_f64(*[b.ptr for b in bufs], 1, 5, 6, 7, scr.ptr, sb, None)
     with pytest.raises(rpx.RpxError):
         L.rpx_mass3dpa_mixed_f64(*[b.ptr for b in bufs], 1, 4, 5, 7, scr.ptr, sb - 256, None)
+
+
+# ---- DIFFUSION3DPA on int8 MFMA ---------------------------------------------------
+
+from test_oracle_kernels import _diffusion_dense, _diffusion_dense_full, _effective_tables  # noqa: E402
+
+
+def _diff_mixed(rpx, bufs, NE, d1d, symmetric=True, nslices=7):
+    L = rpx.lib()
+    sb = L.rpx_diffusion3dpa_mixed_scratch_bytes(d1d, d1d + 1, nslices)
+    assert sb > 0
+    scr = rpx.DeviceBuffer(sb)
+    L.rpx_diffusion3dpa_mixed_f64(*[b.ptr for b in bufs], NE, d1d, d1d + 1, 1 if symmetric else 0, nslices,
+                                  scr.ptr, sb, None)
+
+
+def _diff_magnitude(b, g, D, X, NE, d1d, symmetric):
+    """sum_c |K_c|^T (|O| (|K| |X|))_c with the effective half-storage tables
+    (Base_Seq write order), the magnitude bound of every output."""
+    q1d = d1d + 1
+    Bf, Gf, Bb, Gb = (np.abs(t) for t in _effective_tables(b, g, d1d, q1d))
+    if symmetric:
+        return _diffusion_dense(Bf, Gf, Bb, Gb, np.abs(D), np.abs(X), NE, d1d, q1d)
+    assert np.array_equal(Bf, Bb) and np.array_equal(Gf, Gb)
+    return _diffusion_dense_full(Bf, Gf, np.abs(D), np.abs(X), NE, d1d, q1d)
+
+
+@pytest.mark.parametrize("d1d", [1, 2, 3, 4])
+@pytest.mark.parametrize("bases", ["gll", "random"])
+def test_diffusion3dpa_mixed_meets_1e12(rpx, orc, rng, d1d, bases):
+    """Against the FP64 Base_Seq oracle, relative to the magnitude operator;
+    NE = 37 leaves a partial workgroup of elements.  Random bases break the
+    half-storage symmetry, so only the Base_Seq write order defines the
+    operator; Y starts non-zero (Y +=)."""
+    q1d = d1d + 1
+    NE = 37
+    if bases == "gll":
+        b, g = orc.fem_basis(d1d, q1d)
+    else:
+        b, g = rng.uniform(-1, 1, q1d * d1d), rng.uniform(-1, 1, q1d * d1d)
+    D = orc.fill_sym_tensor(NE, q1d ** 3, 6, 42, 5)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    Y0 = rng.uniform(-1, 1, NE * d1d ** 3)
+    bufs = [dev(rpx, a) for a in (b, g, D, X, Y0)]
+    _diff_mixed(rpx, bufs, NE, d1d)
+    ref = orc.diffusion3dpa(b, g, D, X, Y0, NE, d1d, q1d)
+    mag = _diff_magnitude(b, g, D, X, NE, d1d, True)
+    err = np.max(np.abs(bufs[4].to_array() - ref) / (mag + np.spacing(np.abs(ref))))
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("d1d", [3, 4])
+def test_diffusion3dpa_mixed_full_tensor(rpx, orc, rng, d1d):
+    # nine independent components per point (symmetric = 0)
+    q1d = d1d + 1
+    NE = 50
+    b, g = orc.fem_basis(d1d, q1d)
+    D9 = rng.uniform(-1, 1, NE * 9 * q1d ** 3)
+    X = rng.uniform(-1, 1, NE * d1d ** 3)
+    bufs = [dev(rpx, a) for a in (b, g, D9, X, np.zeros(NE * d1d ** 3))]
+    _diff_mixed(rpx, bufs, NE, d1d, symmetric=False)
+    ref = orc.diffusion3dpa(b, g, D9, X, np.zeros_like(X), NE, d1d, q1d, symmetric=False)
+    mag = _diff_magnitude(b, g, D9, X, NE, d1d, False)
+    err = np.max(np.abs(bufs[4].to_array() - ref) / mag)
+    assert err <= 1e-12, err
+
+
+def test_diffusion3dpa_mixed_config3_size_and_slices(rpx, orc):
+    """NE = 262,144 at (4,5) with the seeded inputs: within 1e-12 of Base_OpenMP
+    relative to the magnitude operator (taken on every 64th element); five
+    slices trade accuracy for speed (about 8 * 2^-35)."""
+    NE, d1d, q1d = 262144, 4, 5
+    b, g = orc.fem_basis(d1d, q1d)
+    D = orc.fill_sym_tensor(NE, q1d ** 3, 6, 42, 2)
+    X = orc.fill_uniform(NE * d1d ** 3, 42, 3, -1.0, 1.0)
+    zeros = np.zeros(NE * d1d ** 3)
+    ref = orc.diffusion3dpa(b, g, D, X, zeros, NE, d1d, q1d, True, orc.OMP)
+    sel = np.arange(0, NE, 64)
+    Dsel = D.reshape(NE, -1)[sel].reshape(-1)
+    Xsel = X.reshape(NE, -1)[sel].reshape(-1)
+    mag = _diff_magnitude(b, g, Dsel, Xsel, sel.size, d1d, True).reshape(sel.size, -1)
+    errs = {}
+    for s in (7, 5):
+        bufs = [dev(rpx, a) for a in (b, g, D, X, zeros)]
+        _diff_mixed(rpx, bufs, NE, d1d, nslices=s)
+        got = bufs[4].to_array().reshape(NE, -1)
+        errs[s] = np.max(np.abs(got[sel] - ref.reshape(NE, -1)[sel]) / mag)
+        assert np.all(np.isfinite(got))
+    assert errs[7] <= 1e-12, errs
+    assert errs[5] <= 1e-8, errs
+
+
+def test_diffusion3dpa_mixed_rejects_bad_arguments(rpx):
+    L = rpx.lib()
+    assert L.rpx_diffusion3dpa_mixed_scratch_bytes(5, 6, 7) == 0
+    assert L.rpx_diffusion3dpa_mixed_scratch_bytes(4, 4, 7) == 0
+    assert L.rpx_diffusion3dpa_mixed_scratch_bytes(4, 5, 4) == 0
+    bufs = [rpx.DeviceBuffer(8 * 6000) for _ in range(5)]
+    sb = L.rpx_diffusion3dpa_mixed_scratch_bytes(4, 5, 7)
+    scr = rpx.DeviceBuffer(sb)
+    with pytest.raises(rpx.RpxError):
+        L.rpx_diffusion3dpa_mixed_f64(*[b.ptr for b in bufs], 4, 4, 5, 1, 7, scr.ptr, sb - 256, None)
+    with pytest.raises(rpx.RpxError):
+        L.rpx_diffusion3dpa_mixed_f64(*[b.ptr for b in bufs], 4, 5, 6, 1, 7, scr.ptr, sb, None)

@@ -84,6 +84,28 @@ def test_scan_in_place_and_segmented(rpx, orc, rng):
         assert np.array_equal(got[b:e], orc.scan(x[b:e]))
 
 
+def test_scan_segmented_then_plain_share_scratch(rpx, orc, rng):
+    """A single-array scan on a freshly initialised scratch that a segmented
+    call has just written (its flags carry tag 0) must not accept those flags,
+    and the segmented form must still be right afterwards."""
+    L = rpx.lib()
+    n = 600_011
+    x = rng.integers(-20, 20, n).astype(np.float64)
+    offs = [0, 7, 70_000, 70_001, 333_333, n]
+    o = rpx.offsets_array(offs)
+    sb = max(L.rpx_scan_segmented_scratch_bytes(o, len(offs) - 1), L.rpx_scan_scratch_bytes(n, None))
+    scr = rpx.DeviceBuffer(sb)
+    L.rpx_scan_scratch_init(scr.ptr, sb, None)
+    dx, y = dev(rpx, x), rpx.DeviceBuffer(8 * n)
+    seg_ref = np.concatenate([orc.scan(x[b:e]) for b, e in zip(offs[:-1], offs[1:])])
+    for _ in range(2):
+        L.rpx_scan_exclusive_segmented_f64(dx.ptr, y.ptr, o, len(offs) - 1, scr.ptr, sb, None)
+        assert np.array_equal(y.to_array(), seg_ref)
+        L.rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, scr.ptr, sb, None, None)
+        assert np.array_equal(y.to_array(), orc.scan(x))
+    assert L.rpx_scan_scratch_status(scr.ptr) == 0
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_scan_segmented_ragged_one_launch(rpx, orc, seed):
     """Ragged segments (empty, odd offsets, multi-tile) in one launch, twice on

@@ -6,6 +6,9 @@ checksum for the same inputs.  The suite's tolerance is 1e-7 absolute; the
 element-wise, stencil, FEM and sort kernels are held to equality of the
 long-double checksum."""
 
+import csv
+import os
+
 import pytest
 
 from rajaperf_amd.suite import run_suite
@@ -182,3 +185,103 @@ def test_harness_mixed_tunings_match_base_seq(rpx, orc, kernel, size, extra, mod
     ref = orc.run(kernel, "Base_Seq", **kw)
     ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
     assert abs(float(run["checksum"] - ref_cs)) <= 1e-12 * max(1.0, abs(float(ref_cs)))
+
+
+MIXED_CASES = [("Apps_MASS3DPA", 125 * 300, []), ("Apps_DIFFUSION3DPA", 64 * 300, []),
+               ("Apps_DIFFUSION3DPA", 125 * 200, ["--d1d", "4"])]
+
+
+@pytest.mark.parametrize("mode", ["reference", "seeded"])
+@pytest.mark.parametrize("kernel,size,extra", MIXED_CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(MIXED_CASES)])
+def test_harness_mixed_tuning_matches_base_seq(rpx, orc, kernel, size, extra, mode):
+    """The "mixed" tuning (int8 MFMA slices of the basis contractions) through
+    KernelBase::execute: the suite's 1e-7 against Base_Seq, and 1e-11
+    relative to the checksum of |Y|-scale."""
+    args = ["-k", kernel, "--size", str(size), "--checkrun", "3", "--input-mode", mode, "-t", "mixed", *extra]
+    res = run_suite(args)
+    assert res["rc"] == 0, res.get("failures")
+    (run,) = [r for r in res["runs"] if r["kernel"] == kernel]
+    assert run["tuning"] == "mixed"
+    kw = dict(size=size, checkrun=3, npasses=1, input_mode=1 if mode == "seeded" else 0)
+    for a, b in zip(extra[::2], extra[1::2]):
+        kw[a.lstrip("-")] = int(b)
+    ref = orc.run(kernel, "Base_Seq", **kw)
+    from fractions import Fraction
+    ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
+    diff = abs(float(run["checksum"] - ref_cs))
+    assert diff < 1e-7, (kernel, float(run["checksum"]), float(ref_cs))
+    assert diff <= 1e-11 * max(1.0, abs(float(ref_cs)))
+
+
+GRAPH_KERNELS = ["Stream_COPY", "Stream_MUL", "Stream_ADD", "Stream_TRIAD", "Stream_DOT", "Algorithm_REDUCE_SUM",
+                 "Algorithm_SCAN", "Polybench_JACOBI_2D", "Polybench_GEMM", "Apps_MASS3DPA", "Apps_DIFFUSION3DPA"]
+
+
+@pytest.mark.parametrize("mode", ["reference", "seeded"])
+def test_harness_graph_tunings_match_base_seq(rpx, orc, mode):
+    """--graph: every tuning also runs as "<tuning>-graph", its rep loop
+    captured in a HIP graph before the timer and replayed inside it, with
+    setUp, checksum and tearDown outside the graph.  Each graph tuning's
+    checksum is held to the oracle's Base_Seq like the eager ones (three
+    passes, so the same rep loop is captured and replayed three times with
+    eager work between); SORT, whose calls upload host tables, has none."""
+    size = 40_000
+    args = ["-k", *GRAPH_KERNELS, "Algorithm_SORT", "--size", str(size), "--checkrun", "3", "--npasses", "3",
+            "--input-mode", mode, "--tsteps", "4", "--graph", "-t", "default", "default-graph", "mixed",
+            "mixed-graph"]
+    res = run_suite(args)
+    assert res["rc"] == 0, res.get("failures")
+    by = {(r["kernel"], r["tuning"]): r for r in res["runs"]}
+    assert ("Algorithm_SORT", "default") in by and ("Algorithm_SORT", "default-graph") not in by
+    for k in GRAPH_KERNELS:
+        g = by[(k, "default-graph")]
+        assert g["passes"] == 3 and g["time_total_s"] > 0
+        kw = dict(size=size, checkrun=3, npasses=3, input_mode=1 if mode == "seeded" else 0)
+        if k == "Polybench_JACOBI_2D":
+            kw["tsteps"] = 4
+        ref = orc.run(k, "Base_Seq", **kw)
+        from fractions import Fraction
+        ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
+        for t in ("default-graph", "mixed-graph"):
+            if (k, t) not in by:
+                continue
+            cs = by[(k, t)]["checksum"]
+            if k in BIT_EXACT and t == "default-graph":
+                assert cs == ref_cs, (k, t)
+            else:
+                assert abs(float(cs - ref_cs)) <= 1e-11 * max(1.0, abs(float(ref_cs))), (k, t)
+    for k in ("Polybench_GEMM", "Apps_MASS3DPA", "Apps_DIFFUSION3DPA"):
+        assert (k, "mixed-graph") in by
+    timing = open(os.path.join(res["outdir"], "RAJAPerf-timing-Average.csv")).read()
+    assert "default-graph" in timing
+
+
+def test_harness_timing_combiners_and_iweighted_checksum(rpx, orc):
+    """--npasses-combiners writes one timing report per combiner (seconds per
+    pass: mean, minimum, maximum over the passes), and every run carries the
+    i-weighted FP64 sum, equal to the oracle's Base_Seq for bit-exact kernels
+    (the same sequential sum over the same values)."""
+    kernels = ["Stream_TRIAD", "Stream_COPY", "Polybench_JACOBI_2D", "Apps_MASS3DPA", "Stream_DOT",
+               "Algorithm_REDUCE_SUM"]
+    res = run_suite(["-k", *kernels, "--size", "30000", "--checkrun", "2", "--npasses", "3", "--input-mode",
+                     "seeded", "--tsteps", "3", "-t", "default", "--npasses-combiners", "Average", "Minimum",
+                     "Maximum"])
+    assert res["rc"] == 0, res.get("failures")
+    tables = {}
+    for c in ("Average", "Minimum", "Maximum"):
+        rows = list(csv.reader(open(os.path.join(res["outdir"], f"RAJAPerf-timing-{c}.csv"))))
+        assert rows[0][0].startswith({"Average": "Mean", "Minimum": "Min", "Maximum": "Max"}[c])
+        tables[c] = {(r[0], r[2]): float(r[3]) for r in rows[2:] if r[3] not in ("", "Not run")}
+    for key, avg in tables["Average"].items():
+        assert tables["Minimum"][key] <= avg <= tables["Maximum"][key]
+    for r in res["runs"]:
+        kw = dict(size=30000, checkrun=2, npasses=3, input_mode=1)
+        if r["kernel"] == "Polybench_JACOBI_2D":
+            kw["tsteps"] = 3
+        ref = orc.run(r["kernel"], "Base_Seq", **kw)
+        if r["kernel"] in BIT_EXACT:
+            assert r["iweighted"] == ref["iweighted"], r["kernel"]
+        else:
+            assert abs(r["iweighted"] - ref["iweighted"]) <= 1e-12 * abs(ref["iweighted"]) * 30000, r["kernel"]
+    report = open(os.path.join(res["outdir"], "RAJAPerf-checksum.txt")).read()
+    assert "i-weighted FP64 sum" in report

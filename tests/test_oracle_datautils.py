@@ -100,6 +100,43 @@ def test_checksum_small_hand_values(orc):
     assert abs(orc.checksum(x, 0.5) - expect) < 1e-14
 
 
+def test_iweighted_sum_closed_forms_and_order(orc, rng):
+    """sum_i (i+1) x[i] in FP64, index order: exact on integer-valued data
+    (n(n+1)/2 for ones; a hand-weighted case), and equal bit for bit to the
+    sequential left-to-right sum; a reordered (reversed) sum differs on data
+    whose rounding depends on order, so the order is pinned."""
+    n = 100_000
+    assert orc.iweighted_sum(np.ones(n)) == n * (n + 1) / 2
+    assert orc.iweighted_sum(np.array([3.0, -1.0, 2.0])) == 3.0 - 2.0 + 6.0
+    assert orc.iweighted_sum(np.array([5.0])) == 5.0 and orc.iweighted_sum(np.array([])) == 0.0
+    x = rng.uniform(-1, 1, 5000)
+    x[::7] *= 1e10
+    seq = 0.0
+    for i, v in enumerate(x):
+        seq += float(i + 1) * float(v)
+    assert orc.iweighted_sum(x) == seq
+    rev = 0.0
+    for i in range(len(x) - 1, -1, -1):
+        rev += float(i + 1) * float(x[i])
+    assert rev != seq
+    exact = sum(Fraction(i + 1) * Fraction(float(v)) for i, v in enumerate(x))
+    mag = sum(Fraction(i + 1) * abs(Fraction(float(v))) for i, v in enumerate(x))
+    assert abs(Fraction(seq) - exact) <= mag * len(x) * Fraction(1, 2**52)
+
+
+def test_suite_run_reports_iweighted_sum(orc):
+    """orc_run's i-weighted checksum is the sum over the checksummed arrays
+    and passes: TRIAD's output array, DOT's scalar per pass."""
+    r = orc.run("Stream_TRIAD", "Base_Seq", input_mode=1, size=1000, checkrun=1)
+    b = orc.fill_uniform(1000, 42, 1, 0.0, 1.0)
+    c = orc.fill_uniform(1000, 42, 2, 0.0, 1.0)
+    assert r["iweighted"] == orc.iweighted_sum(orc.triad(b, c, 0.75))
+    r1 = orc.run("Stream_DOT", "Base_Seq", input_mode=1, size=1000, checkrun=2, npasses=1)
+    r2 = orc.run("Stream_DOT", "Base_Seq", input_mode=1, size=1000, checkrun=2, npasses=3)
+    assert r1["iweighted"] == float(r1["checksum"])
+    assert r2["iweighted"] == 3 * r1["iweighted"]
+
+
 def test_fem_basis_is_lagrange_and_symmetric(orc):
     for d1d in range(2, 9):
         q1d = d1d + 1
