@@ -159,8 +159,9 @@ int rpx_triad_f64(double* a, const double* b, const double* c, double alpha,
  * in index order through two ticket levels (groups of 64 workgroups, then
  * the groups).  Results are identical from run to run for a fixed
  * (n, tune).  x (and b for DOT) must be 16-byte aligned.  `scratch` must be
- * zeroed once before its first use (rpx_reduce_scratch_init); every call
- * leaves it ready for the next. */
+ * zeroed once before its first use (rpx_reduce_scratch_init); every
+ * complete call leaves it zero again, so one scratch serves later calls of
+ * any n and tuning that fit it. */
 size_t rpx_reduce_scratch_bytes(int64_t n, const rpx_tuning* tune);
 int rpx_reduce_scratch_init(void* scratch, size_t scratch_bytes,
                             void* stream);

@@ -18,7 +18,10 @@
 // 256-byte line of its own and no line takes more than max(64, G/64)
 // atomics; with tickets packed 32 to a line, DOT at 2^28 lost 0.17 ms.
 // No floating-point atomics: a given (n, tuning) always produces the same
-// bits.
+// bits.  Every ticket is reset and every partial zeroed once consumed, so a
+// complete call leaves all the scratch it used zero: the ticket slots of a
+// later call with another n or tuning (a larger layout reaches into what
+// were partials) start at zero.
 #include "rpx_common.hpp"
 
 using namespace rpx;
@@ -158,7 +161,11 @@ __global__ __launch_bounds__(BS) void reduce_kernel(const double* __restrict__ a
   // Last workgroup of group g: fixed-order sum of the group's partials.
   if (threadIdx.x < kWave) {
     const int lane = threadIdx.x;
-    double v = (lane < gsize) ? fetch(&partials[gfirst + lane]) : 0.0;
+    double v = 0.0;
+    if (lane < gsize) {
+      v = fetch(&partials[gfirst + lane]);
+      partials[gfirst + lane] = 0.0;  // consumed: the region is left zero for any later layout
+    }
     v = wave_sum(v);
     if (lane == 0) {
       publish(&gpartials[g], v);
@@ -172,7 +179,10 @@ __global__ __launch_bounds__(BS) void reduce_kernel(const double* __restrict__ a
 
   // Last group: fixed-order sum of the group partials.
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < ngroups; i += BS) s += fetch(&gpartials[i]);
+  for (int64_t i = threadIdx.x; i < ngroups; i += BS) {
+    s += fetch(&gpartials[i]);
+    gpartials[i] = 0.0;
+  }
   const double total = block_sum<BS>(s, lds);
   if (threadIdx.x == 0) {
     if (accumulate) result[0] = result[0] + total;

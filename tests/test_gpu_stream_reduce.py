@@ -137,6 +137,30 @@ def test_reduction_ticket_levels_exact(rpx, rng, n, items, flags):
         assert res.to_array()[0] == float(b.astype(np.int64).sum())
 
 
+def test_reduction_scratch_reused_across_layouts(rpx, rng):
+    """One scratch, initialised once, serves calls of different n and
+    tuning.  A layout with more groups than the call before it has ticket
+    slots where that call kept its partials, so each call must leave every
+    partial it consumed at zero (integer data: every sum is exact)."""
+    L = rpx.lib()
+    plan = [(16_777_219, 32), (16_777_219, 8), (1_000_003, 2), (5_000_011, 64), (16_777_219, 2),
+            (300, 8), (16_777_219, 8)]
+    big = max(L.rpx_reduce_scratch_bytes(n, rpx.tuning(0, it)) for n, it in plan)
+    scratch = rpx.DeviceBuffer(big)
+    L.rpx_reduce_scratch_init(scratch.ptr, big, None)
+    a = rng.integers(-1000, 1000, 16_777_219).astype(np.float64)
+    b = rng.integers(-1000, 1000, 16_777_219).astype(np.float64)
+    da, db = dev(rpx, a), dev(rpx, b)
+    res = rpx.DeviceBuffer(8)
+    for n, items in plan:
+        t = rpx.tuning(0, items)
+        L.rpx_fill_const_f64(res.ptr, 1, 0.5, None)
+        L.rpx_dot_f64(da.ptr, db.ptr, n, res.ptr, 0, scratch.ptr, big, t, None)
+        assert res.to_array()[0] == float(np.dot(a[:n].astype(np.int64), b[:n].astype(np.int64))), (n, items)
+        L.rpx_reduce_sum_f64(da.ptr, n, res.ptr, scratch.ptr, big, t, None)
+        assert res.to_array()[0] == float(a[:n].astype(np.int64).sum()), (n, items)
+
+
 def test_reductions_exact_on_integers_and_deterministic(rpx, rng):
     n = 5_000_011
     a = rng.integers(-1000, 1000, n).astype(np.float64)
