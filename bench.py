@@ -303,9 +303,38 @@ def _timed(fn, k):
     return e0.elapsed_ms(e1) / k
 
 
+def _vendor_dgemm_ms(dA, dB, dC, n, alpha, reps):
+    """rocBLAS dgemm on the same row-major operands (C^T = B^T A^T in its
+    column-major terms), the vendor FP64 comparator; None where rocBLAS
+    cannot be loaded."""
+    import ctypes as C
+    try:
+        rb = C.CDLL("/opt/rocm/lib/librocblas.so")
+    except OSError:
+        return None
+    h = C.c_void_p()
+    if rb.rocblas_create_handle(C.byref(h)) != 0:
+        return None
+    rb.rocblas_dgemm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(C.c_double), C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                 C.POINTER(C.c_double), C.c_void_p, C.c_int]
+    al, be = C.c_double(alpha), C.c_double(0.0)
+    none = 111  # rocblas_operation_none
+
+    def run():
+        st = rb.rocblas_dgemm(h, none, none, n, n, n, C.byref(al), dB.ptr, n, dA.ptr, n, C.byref(be), dC.ptr, n)
+        if st != 0:
+            raise RuntimeError(f"rocblas_dgemm status {st}")
+    try:
+        return _timed(run, reps)
+    finally:
+        rb.rocblas_destroy_handle(h)
+
+
 def mixed_vs_fp64(reps: int = 5) -> dict:
-    """The metric's second half, GEMM: 4096^3 (alpha 1.5) on int8 MFMA with 7
-    Ozaki slices against this build's FP64 MFMA kernel and the plain FP64
+    """The metric's second half, GEMM: 4096^3 (alpha 1.5) on int8 MFMA with the
+    modular scheme (13 moduli, 44-bit operands) against this build's FP64
+    MFMA kernel and the plain FP64
     comparator (the suite's Base_HIP shape).  The errors of both paths are
     taken on 8 sampled rows against 80-bit host arithmetic, relative to
     |alpha|(|A||B|)_ij.  Not part of `value`."""
@@ -316,13 +345,15 @@ def mixed_vs_fp64(reps: int = 5) -> dict:
     dA, dB, Cf, Cm = B(8 * n * n), B(8 * n * n), B(8 * n * n), B(8 * n * n)
     L.rpx_fill_uniform_f64(dA.ptr, n * n, SEED, 0, -1.0, 1.0, None)
     L.rpx_fill_uniform_f64(dB.ptr, n * n, SEED, 1, -1.0, 1.0, None)
-    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 7)
+    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 0)
+    bits = L.rpx_gemm_mixed_bits(n, 0)
     scr = B(sb)
     naive = abi.tuning(flags=abi.RPX_TUNE_NAIVE)
     ms_f64 = _timed(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None), reps)
-    ms_mix = _timed(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 7, scr.ptr, sb, None),
+    ms_mix = _timed(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 0, scr.ptr, sb, None),
                     reps)
     ms_plain = _timed(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, naive, None), 1)
+    ms_vendor = _vendor_dgemm_ms(dA, dB, Cf, n, alpha, reps)
     L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None)
     rows = np.sort(np.random.default_rng(0).choice(n, 8, replace=False))
     A = dA.to_array().reshape(n, n)[rows]
@@ -331,9 +362,12 @@ def mixed_vs_fp64(reps: int = 5) -> dict:
     scale = alpha * (np.abs(A) @ np.abs(Bh))
     err_mix = float(np.max(np.abs(Cm.to_array().reshape(n, n)[rows] - ref) / scale))
     err_f64 = float(np.max(np.abs(Cf.to_array().reshape(n, n)[rows] - ref) / scale))
-    return {"kernel": "Polybench_GEMM 4096^3, alpha 1.5, U[-1,1)", "slices": 7, "fp64_mfma_ms": round(ms_f64, 4),
+    return {"kernel": "Polybench_GEMM 4096^3, alpha 1.5, U[-1,1)", "moduli": 13, "operand_bits": bits,
+            "fp64_mfma_ms": round(ms_f64, 4),
             "plain_fp64_ms": round(ms_plain, 3), "mixed_ms": round(ms_mix, 4),
+            "vendor_dgemm_ms": None if ms_vendor is None else round(ms_vendor, 4),
             "speedup_vs_fp64_mfma": round(ms_f64 / ms_mix, 3), "speedup_vs_plain_fp64": round(ms_plain / ms_mix, 2),
+            "speedup_vs_vendor_dgemm": None if ms_vendor is None else round(ms_vendor / ms_mix, 3),
             "max_err_rel_abs_ab_mixed": err_mix, "max_err_rel_abs_ab_fp64": err_f64,
             "error_reference": "80-bit host sums on 8 sampled rows"}
 

@@ -274,21 +274,27 @@ int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha,
                  double beta, int64_t ni, int64_t nj, int64_t nk,
                  const rpx_tuning* tune, void* stream);
 
-/* Mixed-precision GEMM, same semantics: A and B are split into `nslices`
- * error-free fixed-point slices, slice products run on MFMA with exact
- * integer accumulation, and the result is recombined in FP64.  nslices in
- * [1, 8]; 0 = default 7.  Rows of A and columns of B are scaled by powers
- * of two 2^e_i >= max_k |A[i][k]| and 2^f_j >= max_k |B[k][j]|; the error of
- * C[i][j] is below (s+3) 2^-7s |alpha| 2^(e_i+f_j) nk (1.8e-14 |alpha|
- * 2^(e_i+f_j) nk at s = 7), so relative to |alpha|(|A||B|)_ij it grows
- * with the dynamic range within row i and column j.  nk is limited to
- * (2^31-1)/(s 127^2) (19020 at s = 7); larger nk returns
- * RPX_ERR_UNSUPPORTED. */
+/* Mixed-precision GEMM, same semantics, emulated on int8 matrix cores with
+ * the modular (Chinese remainder) scheme: rows of A and columns of B are
+ * scaled by powers of two 2^e_i > max_k |A[i][k]| >= 2^(e_i-1) (likewise
+ * 2^f_j for B) and rounded to b-bit integers; their exact integer product
+ * is assembled from one int8 GEMM per modulus (`nmoduli` of the 16 largest
+ * pairwise coprime integers <= 256) and converted to FP64.  b is the
+ * largest value with 2 nk 4^b below the moduli's product (at most 52;
+ * rpx_gemm_mixed_bits returns it), and the only error besides the FP64
+ * rounding of C is that rounding of A and B:
+ *   |C[i][j] - alpha (AB)[i][j]| <= |alpha| 2^(e_i+f_j-b) nk (1 + 2^-b),
+ * at most 4 2^-b relative to |alpha| max_k|A[i][k]| max_k|B[k][j]| nk, so
+ * it grows with the dynamic range within row i and column j.
+ * nmoduli in [1, 16]; 0 picks the smallest count with b >= 42 (13 moduli,
+ * b = 44 at nk = 4096).  nk is limited to 131071 (RPX_ERR_UNSUPPORTED
+ * above), and to sizes where b >= 1. */
 size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk,
-                                    int nslices);
+                                    int nmoduli);
+int rpx_gemm_mixed_bits(int64_t nk, int nmoduli);
 int rpx_gemm_mixed_f64(double* C, const double* A, const double* B,
                        double alpha, double beta, int64_t ni, int64_t nj,
-                       int64_t nk, int nslices, void* scratch,
+                       int64_t nk, int nmoduli, void* scratch,
                        size_t scratch_bytes, void* stream);
 
 /* ---- apps (partial-assembly finite elements) -------------------------- */

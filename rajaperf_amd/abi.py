@@ -112,6 +112,7 @@ _SIGS = {
     "rpx_jacobi2d_copy_boundary_f64": (_i32, [_vp, _vp, _i64, _vp]),
     "rpx_gemm_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _tp, _vp]),
     "rpx_gemm_mixed_scratch_bytes": (_sz, [_i64, _i64, _i64, _i32]),
+    "rpx_gemm_mixed_bits": (_i32, [_i64, _i32]),
     "rpx_gemm_mixed_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _i32, _vp, _sz, _vp]),
     "rpx_mass3dpa_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _tp, _vp]),
     "rpx_mass3dpa_batched_scratch_bytes": (_sz, [C.POINTER(MassBatch), _i32]),
@@ -162,7 +163,7 @@ class _Lib:
         if not name.startswith("rpx_"):
             raise AttributeError(name)
         fn = getattr(self._dll, name)
-        if name in _SIZE_FUNCS or name in ("rpx_abi_version", "rpx_status_string",
+        if name in _SIZE_FUNCS or name in ("rpx_abi_version", "rpx_status_string", "rpx_gemm_mixed_bits",
                                            "rpx_scan_scratch_status"):
             return fn
 

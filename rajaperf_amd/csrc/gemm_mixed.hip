@@ -1,26 +1,37 @@
 // Mixed-precision GEMM with FP64 accuracy from int8 matrix cores.
 //
 // Same result semantics as the suite's GEMM (src/polybench/POLYBENCH_GEMM.hpp
-// :12-50: C = alpha*A*B, beta never reaches C).  The products run on
-// v_mfma_i32_16x16x64_i8 over error-free fixed-point slices of A and B
-// (the Ozaki splitting):
-//   * every row i of A is scaled by 2^-e_i (e_i from frexp of the row's max
-//     |A|), every column j of B by 2^-f_j, so all entries lie in (-1, 1);
-//   * each scaled entry x is cut into s signed 7-bit digits,
-//       x = sum_{p<s} d_p 2^{-7(p+1)} + r,  |d_p| <= 127,  |r| < 2^{-7s},
-//     by exact FP64 operations (scale by 128, truncate, subtract);
-//   * S_t = sum_{p+q=t} sum_k dA_p[i][k] dB_q[k][j] is an exact int32 sum for
-//     every t < s: level t has at most s digit pairs, so |S_t| <= s*127^2*nk,
-//     below 2^31 for nk <= (2^31-1)/(s*127^2) (19020 at s = 7, 16643 at 8);
-//   * C[i][j] = alpha * 2^(e_i+f_j) * sum_{t<s} S_t 2^{-7(t+2)}, summed in
-//     FP64 from the smallest term up.
-// The dropped digit products are below (s+3) 2^{-7s} relative to
-// 2^(e_i+f_j) per k term; with the default s = 7 the error is below 1e-12
-// relative to |alpha|(|A||B|)_ij.  There are s(s+1)/2 slice products.
+// :12-50: C = alpha*A*B, beta never reaches C).  The product is emulated
+// with the modular integer scheme of Ozaki, Uchino & Imamura ("Ozaki scheme
+// II", 2025): one int8 GEMM per modulus instead of one per pair of
+// fixed-point slices.
+//   * Row i of A is scaled by 2^(b-e_i) and column j of B by 2^(b-f_j),
+//     where 2^(e_i-1) <= max_k |A[i][k]| < 2^e_i (likewise f_j), and rounded
+//     to integers: |A'|, |B'| <= 2^b.  X = A'B' is an exact integer with
+//     |X| <= nk 4^b.
+//   * T pairwise coprime moduli m_t <= 256 (the largest such numbers) with
+//     M = prod m_t > 2 nk 4^b.  For each t the residues of A' and of
+//     y_t B' (y_t = (M/m_t)^-1 mod m_t) lie in [-128, 127] and one GEMM on
+//     v_mfma_i32_16x16x64_i8 gives their product exactly in int32 (nk <=
+//     131071); its epilogue keeps d_t = that product mod m_t = y_t X mod m_t,
+//     one byte per output.
+//   * Chinese remainder theorem: X = sum_t d_t M/m_t (mod M), so
+//     X/M = sum_t d_t/m_t minus the nearest integer, |X/M| < 1/2.  With
+//     1/m_t = h_t + l_t, h_t a multiple of 2^-52, the sum of d_t h_t is kept
+//     exactly in FP64 modulo 1 and the sum of d_t l_t (below T 2^-44) is
+//     added after the integer part is removed, so X comes out with a
+//     relative error of a few 2^-53 however much it cancels.
+//   * C[i][j] = alpha 2^(e_i+f_j-2b) X.
+// The only approximation is the rounding of A and B to b bits below their
+// row/column maxima: |C - alpha AB|_ij <= |alpha| 2^(e_i+f_j-b) nk (1+2^-b)
+// plus FP64 rounding of the result.  b is the largest value with
+// 2 nk 4^b < M (at most 52); by default T is the smallest count giving
+// b >= 42 (13 moduli, b = 44 at nk = 4096), an error below 2^-42 relative
+// to |alpha| max_k|A[i][k]| max_k|B[k][j]| nk.
 //
-// Scratch holds the int8 slices (A: [p][i][k], B: [q][j][k], K contiguous,
-// zero-padded to tile multiples), the per-row/column exponents and the
-// int32 partial sums S_t ([t][i][j] over the padded grid).
+// Scratch holds the int8 residues (A: [t][i][k], B: [t][j][k], K
+// contiguous, zero-padded to tile multiples), the per-row/column exponents
+// and the residue bytes d_t ([t] then the GEMM's accumulator fragments).
 #include "rpx_common.hpp"
 
 using namespace rpx;
@@ -28,33 +39,107 @@ using namespace rpx;
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef double v2d __attribute__((ext_vector_type(2)));
-typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int kMaxSlices = 8;
+constexpr int kMaxMods = 16;
+// The 16 largest pairwise coprime integers not above 256.
+constexpr int kMods[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
+constexpr int kMinBits = 42;       // automatic modulus count: the smallest T with b >= 42
+constexpr int64_t kMaxK = 131071;  // 128 * 128 * nk < 2^31
+
 constexpr int kTM = 256, kTN = 256, kTK = 128;  // block tile, K bytes per stage
 constexpr int kChunks = kTK / 16;               // 16-byte chunks per tile row and stage
-// Largest nk for which every S_t stays inside int32 with s slices.
-constexpr int64_t max_k(int s) { return int64_t(INT32_MAX) / (int64_t(s) * 127 * 127); }
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Per-call constants of the modular scheme, passed by value to the kernels.
+struct CrtParams {
+  int T, b;
+  double m[kMaxMods];    // moduli
+  double inv[kMaxMods];  // fl(1/m_t), used only to pick a quotient that is then corrected
+  double lo[kMaxMods];   // smallest symmetric residue, -floor(m_t/2)
+  double y[kMaxMods];    // (M/m_t)^-1 mod m_t
+  double h[kMaxMods];    // floor(2^52/m_t) 2^-52
+  double l[kMaxMods];    // 1/m_t - h_t, rounded
+  double M_hi, M_lo;     // prod m_t = M_hi + M_lo, M_lo rounded (a 2^-106 relative error)
+};
+
+typedef unsigned __int128 u128;
+
+int64_t inverse_mod(int64_t a, int64_t m) {
+  int64_t t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    const int64_t q = r / nr;
+    int64_t tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  return t < 0 ? t + m : t;
+}
+
+// Largest b <= 52 with 2 nk 4^b < M, or -1.
+int bits_for(int64_t nk, int T) {
+  u128 M = 1;
+  for (int t = 0; t < T; ++t) M *= u128(kMods[t]);
+  const u128 k = u128(nk > 0 ? nk : 1);
+  for (int b = 52; b >= 0; --b)
+    if (2 * k * (u128(1) << (2 * b)) < M) return b;
+  return -1;
+}
+
+int moduli_for(int64_t nk, int nmoduli) {
+  if (nmoduli > 0) return nmoduli;
+  for (int T = 1; T <= kMaxMods; ++T)
+    if (bits_for(nk, T) >= kMinBits) return T;
+  return kMaxMods;
+}
+
+bool crt_setup(int64_t nk, int nmoduli, CrtParams* p) {
+  const int T = moduli_for(nk, nmoduli);
+  if (T < 1 || T > kMaxMods) return false;
+  const int b = bits_for(nk, T);
+  if (b < 1) return false;
+  p->T = T;
+  p->b = b;
+  u128 M = 1;
+  for (int t = 0; t < T; ++t) M *= u128(kMods[t]);
+  // M < 2^126 fits long double's range; hi is an integer, so hiu = hi exactly
+  // and M - hi (below 2^73 in magnitude) is exact in 128-bit integers
+  // before its rounding to FP64.
+  const double hi = double(static_cast<long double>(M));
+  const u128 hiu = static_cast<u128>(static_cast<long double>(hi));
+  p->M_hi = hi;
+  p->M_lo = hiu >= M ? -double(static_cast<long double>(hiu - M)) : double(static_cast<long double>(M - hiu));
+  for (int t = 0; t < kMaxMods; ++t) {
+    const int64_t m = kMods[t];
+    p->m[t] = double(m);
+    p->inv[t] = 1.0 / double(m);
+    p->lo[t] = -double(m / 2);
+    int64_t mt = 1;  // (M / m_t) mod m_t
+    for (int s = 0; s < T; ++s)
+      if (s != t) mt = mt * kMods[s] % m;
+    p->y[t] = t < T ? double(inverse_mod(mt, m)) : 0.0;
+    const uint64_t two52 = uint64_t(1) << 52;
+    p->h[t] = std::ldexp(double(two52 / uint64_t(m)), -52);
+    p->l[t] = std::ldexp(double(two52 % uint64_t(m)) / double(m), -52);
+  }
+  return true;
+}
+
 struct MixedLayout {
   int64_t nip, njp, nkp;
-  size_t a_off, b_off, ea_off, fb_off, s_off, total;
+  size_t a_off, b_off, ea_off, fb_off, r_off, total;
 };
-MixedLayout mixed_layout(int64_t ni, int64_t nj, int64_t nk, int s) {
+MixedLayout mixed_layout(int64_t ni, int64_t nj, int64_t nk, int T) {
   MixedLayout l;
   l.nip = round_up(ni, kTM);
   l.njp = round_up(nj, kTN);
   l.nkp = round_up(nk > 0 ? nk : 1, kTK);
   l.a_off = 0;
-  l.b_off = align256(size_t(s) * l.nip * l.nkp);
-  l.ea_off = l.b_off + align256(size_t(s) * l.njp * l.nkp);
+  l.b_off = align256(size_t(T) * l.nip * l.nkp);
+  l.ea_off = l.b_off + align256(size_t(T) * l.njp * l.nkp);
   l.fb_off = l.ea_off + align256(size_t(l.nip) * 4);
-  l.s_off = l.fb_off + align256(size_t(l.njp) * 4);
-  l.total = l.s_off + align256(size_t(s) * l.nip * l.njp * 4);  // S_t, int32, [t][i][j]
+  l.r_off = l.fb_off + align256(size_t(l.njp) * 4);
+  l.total = l.r_off + align256(size_t(T) * l.nip * l.njp);  // d_t, one byte per output
   return l;
 }
 
@@ -64,26 +149,22 @@ __device__ __forceinline__ int exponent_of(double m) {
   return e;
 }
 
-// Cuts x (|x| < 1) into s digits; digits[p] in [-127, 127].
-template <int S>
-__device__ __forceinline__ void cut(double x, int* digits) {
-#pragma unroll
-  for (int p = 0; p < S; ++p) {
-    const double t = x * 128.0;
-    const double d = trunc(t);
-    digits[p] = int(d);
-    x = t - d;
-  }
+// Symmetric residue of the integer-valued x (|x| <= 2^52) modulo m: the
+// quotient from fl(1/m) is within one of the true one, the fma is exact.
+__device__ __forceinline__ double sym_mod(double x, double m, double inv, double lo) {
+  const double q = rint(x * inv);
+  double r = fma(-q, m, x);
+  if (r < lo) r += m;
+  if (r >= lo + m) r -= m;
+  return r;
 }
 
-// One workgroup per (padded) row of A: row exponent, then the slices.  Loads
-// are unrolled and unconditional wherever a whole group lies inside the row,
-// so each work-item keeps 8 or 16 loads in flight instead of waiting on one
-// load at a time.
-template <int S>
+// One workgroup per (padded) row of A: row exponent, then the residues.
+// Each work-item takes 16 consecutive k and writes one 16-byte chunk per
+// modulus.
 __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restrict__ A, int64_t ni, int64_t nk,
                                                          int64_t nkp, int64_t nip, signed char* __restrict__ As,
-                                                         int* __restrict__ ea) {
+                                                         int* __restrict__ ea, const CrtParams P) {
   __shared__ double red[4];
   const int64_t i = blockIdx.x;
   const int tid = threadIdx.x;
@@ -108,7 +189,6 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
   const int e = exponent_of(m);
   if (tid == 0) ea[i] = e;
-  // 16 consecutive k per work-item per step; one 16-byte store per slice.
   for (int64_t k0 = int64_t(tid) * 16; k0 < nkp; k0 += 256 * 16) {
     double x[16];
     if (live && k0 + 16 <= nk) {
@@ -118,17 +198,21 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
 #pragma unroll
       for (int u = 0; u < 16; ++u) x[u] = (live && k0 + u < nk) ? row[k0 + u] : 0.0;
     }
-    int dig[16][S];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) cut<S>(ldexp(x[u], -e), dig[u]);
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
+    for (int u = 0; u < 16; ++u) x[u] = rint(ldexp(x[u], P.b - e));
+    for (int t = 0; t < P.T; ++t) {
       v4i w;
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        w[c] = (dig[4 * c][p] & 0xFF) | ((dig[4 * c + 1][p] & 0xFF) << 8) | ((dig[4 * c + 2][p] & 0xFF) << 16) |
-               ((dig[4 * c + 3][p] & 0xFF) << 24);
-      *reinterpret_cast<v4i*>(As + (int64_t(p) * nip + i) * nkp + k0) = w;
+      for (int c = 0; c < 4; ++c) {
+        int packed = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = int(sym_mod(x[4 * c + q], P.m[t], P.inv[t], P.lo[t]));
+          packed |= (r & 0xFF) << (8 * q);
+        }
+        w[c] = packed;
+      }
+      *reinterpret_cast<v4i*>(As + (int64_t(t) * nip + i) * nkp + k0) = w;
     }
   }
 }
@@ -159,18 +243,19 @@ __global__ void fill_int_kernel(int* p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
-// B -> Bs[q][j][k] through an LDS transpose: a workgroup owns 64 columns x 64 k.
-template <int S>
+// B -> Bs[t][j][k] through an LDS transpose: a workgroup owns 64 columns x
+// 64 k and writes one modulus at a time.  The residues are those of
+// y_t B', so the GEMM's product is y_t X modulo m_t.
 __global__ __launch_bounds__(256) void split_cols_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
                                                          int64_t nkp, int64_t njp, const int* __restrict__ fb,
-                                                         signed char* __restrict__ Bs) {
-  __shared__ __attribute__((aligned(16))) signed char lds[S][64][64 + 16];
+                                                         signed char* __restrict__ Bs, const CrtParams P) {
+  __shared__ __attribute__((aligned(16))) signed char lds[64][64 + 16];
   const int64_t j0 = int64_t(blockIdx.x) * 64, k0 = int64_t(blockIdx.y) * 64;
   const int tid = threadIdx.x;
   const int jj = tid & 63;
   const int64_t j = j0 + jj;
   const int f = (j < nj) ? fb[j] : 0;
-  const int sh = -(f == INT32_MIN ? 0 : f);
+  const int sh = P.b - (f == INT32_MIN ? 0 : f);
   double x[16];
   if (j0 + 64 <= nj && k0 + 64 <= nk) {
 #pragma unroll
@@ -183,19 +268,19 @@ __global__ __launch_bounds__(256) void split_cols_kernel(const double* __restric
     }
   }
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    int dig[S];
-    cut<S>(ldexp(x[u], sh), dig);
-#pragma unroll
-    for (int q = 0; q < S; ++q) lds[q][jj][(tid >> 6) + 4 * u] = (signed char)dig[q];
-  }
-  __syncthreads();
-  // 64 columns x 64 bytes per slice: 16 bytes per work-item.
+  for (int u = 0; u < 16; ++u) x[u] = rint(ldexp(x[u], sh));
+  // 64 columns x 64 bytes per modulus: 16 bytes per work-item.
   const int row = tid >> 2, part = (tid & 3) * 16;
+  for (int t = 0; t < P.T; ++t) {
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    const v4i w = *reinterpret_cast<const v4i*>(&lds[q][row][part]);
-    *reinterpret_cast<v4i*>(Bs + (int64_t(q) * njp + j0 + row) * nkp + k0 + part) = w;
+    for (int u = 0; u < 16; ++u) {
+      const double r = sym_mod(x[u], P.m[t], P.inv[t], P.lo[t]);
+      lds[jj][(tid >> 6) + 4 * u] = (signed char)int(sym_mod(r * P.y[t], P.m[t], P.inv[t], P.lo[t]));
+    }
+    __syncthreads();
+    const v4i w = *reinterpret_cast<const v4i*>(&lds[row][part]);
+    *reinterpret_cast<v4i*>(Bs + (int64_t(t) * njp + j0 + row) * nkp + k0 + part) = w;
+    __syncthreads();
   }
 }
 
@@ -210,26 +295,25 @@ __device__ __forceinline__ int swz(int r, int c) {
 static_assert(kChunks == 16 || kChunks == 8, "swizzle covers 8 or 16 chunks per row");
 
 // 256x256 output tile per workgroup of eight wavefronts (64x128 each, a 4x8
-// grid of 16x16 int32 accumulators).  For t = s-1 down to 0 the K loop runs
-// over the concatenation of the slice pairs (p, t-p), and the exact int32
-// sum S_t of the tile goes to scratch; ozaki_combine_kernel folds the S_t
-// into C.  The tile size is set by L2 bandwidth: a stage brings 128 bytes of
-// K for 256 rows of A and of B (64 KiB) for 8.4 M MACs, 128 MACs per byte,
-// where a 128x128 tile's 64 left the MFMAs waiting on L2.  Keeping only the
-// int32 accumulators in registers (an FP64 fold would take two more per
-// output) is what lets a wavefront own 64x128 outputs at two wavefronts per
-// SIMD.  Two stages of LDS (128 KiB), one workgroup per CU; the next stage's
-// global loads are in flight in registers while one computes.
+// grid of 16x16 int32 accumulators).  For each modulus the K loop runs over
+// that modulus's residues, and the epilogue reduces each exact int32 sum
+// modulo m_t and stores the byte.  The tile size is set by L2 bandwidth: a
+// stage brings 128 bytes of K for 256 rows of A and of B (64 KiB) for
+// 8.4 M MACs, 128 MACs per byte, where a 128x128 tile's 64 left the MFMAs
+// waiting on L2.  Two stages of LDS (128 KiB), one workgroup per CU; the
+// next stage's global loads are in flight in registers while one computes.
 constexpr int kThreads = 512;
 constexpr int kWaveM = 64, kWaveN = 128;
 constexpr int kFm = kWaveM / 16, kFn = kWaveN / 16;  // 16x16 tiles per wavefront
+constexpr int kFrags = (kThreads / kWave) * kFm * kFn;  // accumulator fragments per workgroup tile
 constexpr int kLoads = kTM * kChunks / kThreads;       // 16-byte chunks per work-item per operand
 static_assert(kTM == kTN, "one loader mapping for both operands");
 
-template <int S>
-__global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(int* __restrict__ Sout, const signed char* __restrict__ As,
+__global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restrict__ R,
+                                                              const signed char* __restrict__ As,
                                                               const signed char* __restrict__ Bs, int64_t nip,
-                                                              int64_t njp, int64_t nkp, int tiles_n) {
+                                                              int64_t njp, int64_t nkp, int tiles_n,
+                                                              const CrtParams P) {
   __shared__ v4i Al[2][kTM * kChunks];
   __shared__ v4i Bl[2][kTN * kChunks];
   int64_t tm, tn;
@@ -242,28 +326,26 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(int* __restrict__ 
   constexpr int kRowStep = kThreads / kChunks;
   const int lc = tid % kChunks, lr = tid / kChunks;
   const int fr = lane & 15, fk = lane >> 4;  // MFMA operand row and k quarter
+  // This wavefront's fragments in the residue array: [tile][wave][i][j][lane].
+  const int64_t frag0 = ((tm * tiles_n + tn) * (kThreads / kWave) + wave) * (kFm * kFn);
+  const int64_t per_mod = nip * njp / 4;  // 32-bit words per modulus
 
-  for (int t = S - 1; t >= 0; --t) {
-    const int p_lo = t - (S - 1) > 0 ? t - (S - 1) : 0;
-    const int p_hi = t < S - 1 ? t : S - 1;
-    const int64_t nsteps = int64_t(p_hi - p_lo + 1) * ksteps;
+  for (int t = 0; t < P.T; ++t) {
     v4i acc[kFm][kFn];
 #pragma unroll
     for (int i = 0; i < kFm; ++i)
 #pragma unroll
       for (int j = 0; j < kFn; ++j) acc[i][j] = v4i{0, 0, 0, 0};
 
+    const signed char* at = As + (int64_t(t) * nip + m0 + lr) * nkp + 16 * lc;
+    const signed char* bt = Bs + (int64_t(t) * njp + n0 + lr) * nkp + 16 * lc;
     v4i ga[kLoads], gb[kLoads];
     auto gload = [&](int64_t st) {
-      const int p = p_lo + int(st / ksteps);
-      const int q = t - p;
-      const int64_t k0 = (st % ksteps) * kTK;
-      const signed char* a = As + (int64_t(p) * nip + m0 + lr) * nkp + k0 + 16 * lc;
-      const signed char* b = Bs + (int64_t(q) * njp + n0 + lr) * nkp + k0 + 16 * lc;
+      const int64_t k0 = st * kTK;
 #pragma unroll
       for (int u = 0; u < kLoads; ++u) {
-        ga[u] = *reinterpret_cast<const v4i*>(a + int64_t(kRowStep * u) * nkp);
-        gb[u] = *reinterpret_cast<const v4i*>(b + int64_t(kRowStep * u) * nkp);
+        ga[u] = *reinterpret_cast<const v4i*>(at + int64_t(kRowStep * u) * nkp + k0);
+        gb[u] = *reinterpret_cast<const v4i*>(bt + int64_t(kRowStep * u) * nkp + k0);
       }
     };
     auto lstore = [&](int buf) {
@@ -275,16 +357,15 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(int* __restrict__ 
     };
 
     gload(0);
-    __syncthreads();  // previous t's readers are done with buffer 0
+    __syncthreads();  // the previous modulus's readers are done with buffer 0
     lstore(0);
     __syncthreads();
-    for (int64_t st = 0; st < nsteps; ++st) {
+    for (int64_t st = 0; st < ksteps; ++st) {
       const int buf = int(st & 1);
-      if (st + 1 < nsteps) gload(st + 1);
+      if (st + 1 < ksteps) gload(st + 1);
       // Raised wave priority over the MFMA block: with two wavefronts per
       // SIMD the arbiter then keeps feeding the matrix pipe from one while
-      // the other issues its loads and LDS stores (7 slices at 4096^3:
-      // 2.05-2.10 -> 2.00-2.01 ms; the FP64 MFMA kernel was slower with it).
+      // the other issues its loads and LDS stores.
       __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int kk = 0; kk < kChunks / 4; ++kk) {
@@ -301,102 +382,134 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(int* __restrict__ 
             acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
       }
       __builtin_amdgcn_s_setprio(0);
-      if (st + 1 < nsteps) {
+      if (st + 1 < ksteps) {
         lstore(buf ^ 1);
         __syncthreads();
       }
     }
-    // int32 16x16 accumulator layout: element r of lane l is row 4*(l/16)+r,
-    // column l%16.  The padded S_t grid covers every tile.
-    int* st_out = Sout + int64_t(t) * nip * njp;
+    // d_t = sum mod m_t in [0, m_t): |sum| < 2^31, so the fma is exact and
+    // the quotient from fl(1/m_t) is off by at most one.  The four rows of
+    // a lane's accumulator go to one 32-bit word.
+    const double m = P.m[t], inv = P.inv[t];
+    unsigned* rt = R + int64_t(t) * per_mod + frag0 * kWave + lane;
 #pragma unroll
     for (int i = 0; i < kFm; ++i)
 #pragma unroll
-      for (int j = 0; j < kFn; ++j)
+      for (int j = 0; j < kFn; ++j) {
+        unsigned w = 0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t gi = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-          const int64_t gj = n0 + wn + j * 16 + (lane & 15);
-          st_out[gi * njp + gj] = acc[i][j][r];
+          const double c = double(acc[i][j][r]);
+          double d = fma(-floor(c * inv), m, c);
+          if (d < 0.0) d += m;
+          if (d >= m) d -= m;
+          w |= unsigned(int(d)) << (8 * r);
         }
+        rt[(i * kFn + j) * kWave] = w;
+      }
   }
 }
 
-// C[i][j] = alpha * 2^(e_i+f_j) * sum_t S_t[i][j] 2^{-7(t+2)}, the terms
-// added in FP64 from t = s-1 (the smallest) down to 0.
-template <int S>
-__global__ __launch_bounds__(256) void ozaki_combine_kernel(double* __restrict__ C, const int* __restrict__ Sin,
-                                                            const int* __restrict__ ea, const int* __restrict__ fb,
-                                                            double alpha, int64_t ni, int64_t nj, int64_t nip,
-                                                            int64_t njp) {
-  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (idx >= ni * nj) return;
-  const int64_t i = idx / nj, j = idx % nj;
-  double v = 0.0;
+// One work-item per (fragment, lane): the four outputs of that accumulator
+// slot.  X/M = sum_t d_t (h_t + l_t) mod 1; the h_t part is exact (every
+// term is a multiple of 2^-52 below 1 and the sum is reduced modulo 1 after
+// each add).
+__global__ __launch_bounds__(256) void crt_combine_kernel(double* __restrict__ C, const unsigned* __restrict__ R,
+                                                          const int* __restrict__ ea, const int* __restrict__ fb,
+                                                          double alpha, int64_t ni, int64_t nj, int64_t nip,
+                                                          int64_t njp, int tiles_n, const CrtParams P) {
+  const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t per_mod = nip * njp / 4;
+  if (g >= per_mod) return;
+  const int lane = int(g % kWave);
+  const int64_t frag = g / kWave;
+  const int64_t tile = frag / kFrags;
+  const int f = int(frag % kFrags);
+  const int wave = f / (kFm * kFn), i = (f / kFn) % kFm, j = f % kFn;
+  const int64_t row0 = (tile / tiles_n) * kTM + (wave >> 1) * kWaveM + i * 16 + 4 * (lane >> 4);
+  const int64_t col = (tile % tiles_n) * kTN + (wave & 1) * kWaveN + j * 16 + (lane & 15);
+  double fh[4] = {0.0, 0.0, 0.0, 0.0}, fl[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < P.T; ++t) {
+    const unsigned w = R[int64_t(t) * per_mod + g];
+    const double h = P.h[t], l = P.l[t];
 #pragma unroll
-  for (int t = S - 1; t >= 0; --t) v += double(Sin[(int64_t(t) * nip + i) * njp + j]) * ldexp(1.0, -7 * (t + 2));
-  const int fj = fb[j];
-  C[idx] = alpha * ldexp(v, ea[i] + (fj == INT32_MIN ? 0 : fj));
-}
-
-template <int S>
-int run_mixed(double* C, const double* A, const double* B, double alpha, int64_t ni, int64_t nj, int64_t nk,
-              const MixedLayout& l, char* base, hipStream_t s) {
-  signed char* As = reinterpret_cast<signed char*>(base + l.a_off);
-  signed char* Bs = reinterpret_cast<signed char*>(base + l.b_off);
-  int* ea = reinterpret_cast<int*>(base + l.ea_off);
-  int* fb = reinterpret_cast<int*>(base + l.fb_off);
-  split_rows_kernel<S><<<dim3(unsigned(l.nip)), dim3(256), 0, s>>>(A, ni, nk, l.nkp, l.nip, As, ea);
-  fill_int_kernel<<<dim3(unsigned(ceil_div(l.njp, 256))), dim3(256), 0, s>>>(fb, l.njp, INT32_MIN);
-  const int64_t kchunk = 64;
-  if (nk > 0)
-    col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 256)), unsigned(ceil_div(nk, kchunk))), dim3(256), 0, s>>>(
-        B, nk, nj, kchunk, fb);
-  split_cols_kernel<S><<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 64)), dim3(256), 0, s>>>(B, nk, nj, l.nkp,
-                                                                                              l.njp, fb, Bs);
-  int* St = reinterpret_cast<int*>(base + l.s_off);
-  const int64_t tm = l.nip / kTM, tn = l.njp / kTN;
-  ozaki_gemm_kernel<S><<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(St, As, Bs, l.nip, l.njp, l.nkp, int(tn));
-  ozaki_combine_kernel<S><<<dim3(unsigned(ceil_div(ni * nj, 256))), dim3(256), 0, s>>>(C, St, ea, fb, alpha, ni, nj,
-                                                                                       l.nip, l.njp);
-  return launch_status();
+    for (int r = 0; r < 4; ++r) {
+      const double d = double((w >> (8 * r)) & 0xFFu);
+      const double s = fma(d, h, fh[r]);  // exact: both terms are multiples of 2^-52 below 1
+      fh[r] = s - floor(s);
+      fl[r] = fma(d, l, fl[r]);
+    }
+  }
+  if (col >= nj) return;
+  const int fj = fb[col];
+  const int sc = (fj == INT32_MIN ? 0 : fj) - 2 * P.b;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t row = row0 + r;
+    if (row >= ni) break;
+    // X = (fx + fl) M with fx = fh - nearest integer (exact, a multiple of
+    // 2^-52 in [-1/2, 1/2]) and M = M_hi + M_lo: the product fx M_hi is
+    // split exactly by an fma, the small terms are added to its error.
+    double fx = fh[r] - rint(fh[r]);
+    if (fx + fl[r] > 0.5) fx -= 1.0;
+    if (fx + fl[r] < -0.5) fx += 1.0;
+    const double p = fx * P.M_hi;
+    const double e = fma(fx, P.M_hi, -p);
+    const double x = p + (e + fma(fx, P.M_lo, fl[r] * P.M_hi));
+    C[row * nj + col] = alpha * ldexp(x, ea[row] + sc);
+  }
 }
 
 }  // namespace
 
 extern "C" {
 
-size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk, int nslices) {
-  if (ni < 0 || nj < 0 || nk < 0) return 0;
-  const int s = nslices <= 0 ? 7 : nslices;
-  if (s > kMaxSlices) return 0;
-  return mixed_layout(ni, nj, nk, s).total;
+size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk, int nmoduli) {
+  if (ni < 0 || nj < 0 || nk < 0 || nmoduli < 0 || nmoduli > kMaxMods || nk > kMaxK) return 0;
+  CrtParams P;
+  if (!crt_setup(nk, nmoduli, &P)) return 0;
+  return mixed_layout(ni, nj, nk, P.T).total;
+}
+
+int rpx_gemm_mixed_bits(int64_t nk, int nmoduli) {
+  if (nk < 0 || nk > kMaxK || nmoduli < 0 || nmoduli > kMaxMods) return RPX_ERR_INVALID_ARGUMENT;
+  CrtParams P;
+  if (!crt_setup(nk, nmoduli, &P)) return RPX_ERR_UNSUPPORTED;
+  return P.b;
 }
 
 int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha, double /*beta*/, int64_t ni,
-                       int64_t nj, int64_t nk, int nslices, void* scratch, size_t scratch_bytes, void* stream) {
-  if (ni < 0 || nj < 0 || nk < 0 || nslices < 0 || nslices > kMaxSlices) return RPX_ERR_INVALID_ARGUMENT;
+                       int64_t nj, int64_t nk, int nmoduli, void* scratch, size_t scratch_bytes, void* stream) {
+  if (ni < 0 || nj < 0 || nk < 0 || nmoduli < 0 || nmoduli > kMaxMods) return RPX_ERR_INVALID_ARGUMENT;
   if (ni == 0 || nj == 0) return RPX_SUCCESS;
   if (!C || !scratch || (nk > 0 && (!A || !B))) return RPX_ERR_INVALID_ARGUMENT;
-  const int s = nslices == 0 ? 7 : nslices;
-  if (nk > max_k(s)) return RPX_ERR_UNSUPPORTED;
-  const MixedLayout l = mixed_layout(ni, nj, nk, s);
+  if (nk > kMaxK) return RPX_ERR_UNSUPPORTED;
+  CrtParams P;
+  if (!crt_setup(nk, nmoduli, &P)) return RPX_ERR_UNSUPPORTED;
+  const MixedLayout l = mixed_layout(ni, nj, nk, P.T);
   if (scratch_bytes < l.total) return RPX_ERR_SCRATCH_TOO_SMALL;
-  if (l.nip > INT32_MAX || (l.nip / kTM) * (l.njp / kTN) > INT32_MAX || ceil_div(ni * nj, 256) > INT32_MAX)
+  const int64_t tm = l.nip / kTM, tn = l.njp / kTN;
+  if (l.nip > INT32_MAX || tm * tn > INT32_MAX || ceil_div(l.nip * l.njp / 4, 256) > INT32_MAX)
     return RPX_ERR_UNSUPPORTED;
   char* base = reinterpret_cast<char*>(scratch);
-  hipStream_t st = as_stream(stream);
-  switch (s) {
-    case 1: return run_mixed<1>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 2: return run_mixed<2>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 3: return run_mixed<3>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 4: return run_mixed<4>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 5: return run_mixed<5>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 6: return run_mixed<6>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 7: return run_mixed<7>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    case 8: return run_mixed<8>(C, A, B, alpha, ni, nj, nk, l, base, st);
-    default: return RPX_ERR_UNSUPPORTED;
-  }
+  signed char* As = reinterpret_cast<signed char*>(base + l.a_off);
+  signed char* Bs = reinterpret_cast<signed char*>(base + l.b_off);
+  int* ea = reinterpret_cast<int*>(base + l.ea_off);
+  int* fb = reinterpret_cast<int*>(base + l.fb_off);
+  unsigned* R = reinterpret_cast<unsigned*>(base + l.r_off);
+  hipStream_t s = as_stream(stream);
+  split_rows_kernel<<<dim3(unsigned(l.nip)), dim3(256), 0, s>>>(A, ni, nk, l.nkp, l.nip, As, ea, P);
+  fill_int_kernel<<<dim3(unsigned(ceil_div(l.njp, 256))), dim3(256), 0, s>>>(fb, l.njp, INT32_MIN);
+  const int64_t kchunk = 64;
+  if (nk > 0)
+    col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 256)), unsigned(ceil_div(nk, kchunk))), dim3(256), 0, s>>>(
+        B, nk, nj, kchunk, fb);
+  split_cols_kernel<<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 64)), dim3(256), 0, s>>>(B, nk, nj, l.nkp, l.njp,
+                                                                                           fb, Bs, P);
+  ozaki_gemm_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(R, As, Bs, l.nip, l.njp, l.nkp, int(tn), P);
+  crt_combine_kernel<<<dim3(unsigned(ceil_div(l.nip * l.njp / 4, 256))), dim3(256), 0, s>>>(
+      C, R, ea, fb, alpha, ni, nj, l.nip, l.njp, int(tn), P);
+  return launch_status();
 }
 
 }  // extern "C"

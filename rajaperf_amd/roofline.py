@@ -197,19 +197,21 @@ def config4(rows, reps, sample_rows=8):
     err_f64 = float(np.max(np.abs(C64 - ref) / scale))
     _row(rows, "Polybench_GEMM(fp64-mfma)", 4, ms_f64, None, flops, "mfma", FP64_TFLOPS, err_vs_exact=err_f64)
     _row(rows, "Polybench_GEMM(fp64-naive)", 4, ms_naive, None, flops, "mfma", FP64_TFLOPS)
-    # s slices: s(s+1)/2 int8 slice products; the error bound is (s+3) 2^-7s
-    # relative to |alpha|(|A||B|)_ij (7 slices: 1.8e-14, 6 slices: 2.0e-12).
-    for s_ in (7, 6):
-        sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, s_)
+    # T moduli: T int8 GEMMs; the error bound is 2^(2-b) relative to
+    # |alpha| rowmax colmax nk, b from the library (13 moduli: b = 44,
+    # 12 moduli: b = 40 at nk = 4096).
+    for t_ in (0, 12):
+        sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, t_)
+        bits = L.rpx_gemm_mixed_bits(n, t_)
+        nmod = 13 if t_ == 0 else t_
         scr = B(sb)
-        ms_mix = _time(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, s_, scr.ptr, sb,
+        ms_mix = _time(lambda: L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, t_, scr.ptr, sb,
                                                     None), reps)
         Cmix = Cm.to_array().reshape(n, n)[rows_idx]
         err_mix = float(np.max(np.abs(Cmix - ref) / scale))
-        pairs = s_ * (s_ + 1) // 2
-        _row(rows, f"Polybench_GEMM(mixed-int8x{s_})", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
-             err_vs_exact=err_mix, err_bound=(s_ + 3) * 2.0 ** (-7 * s_), slice_products=pairs,
-             int8_tops=round(pairs * flops / (ms_mix / 1e3) / 1e12, 1),
+        _row(rows, f"Polybench_GEMM(mixed-int8-{nmod}mod)", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
+             err_vs_exact=err_mix, err_bound=2.0 ** (2 - bits), int8_gemms=nmod,
+             int8_tops=round(nmod * flops / (ms_mix / 1e3) / 1e12, 1),
              speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
         del scr
     del dA, dB, Cf, Cm

@@ -37,7 +37,7 @@ POLYBENCH_GEMM::POLYBENCH_GEMM(const RunParams& params) : KernelBase(Polybench_G
 void POLYBENCH_GEMM::setMI355XTuningDefinitions(VariantID vid) {
   addVariantTuningName(vid, "default");  // FP64 on the MFMA f64 units
   addVariantTuningName(vid, "naive");    // one thread per C element, the plain FP64 comparator
-  addVariantTuningName(vid, "mixed");    // int8 MFMA on 7 Ozaki slices, FP64 accuracy (error <= 1.8e-14 |A||B|)
+  addVariantTuningName(vid, "mixed");    // int8 MFMA, one GEMM per modulus (13 at nk = 4096), FP64 result
 }
 void POLYBENCH_GEMM::setUp(VariantID, size_t) {
   allocAndInitData(m_A, Size_type(m_ni * m_nk), -1.0, 1.0);
@@ -54,12 +54,12 @@ void POLYBENCH_GEMM::tearDown(VariantID, size_t) {
 }
 void POLYBENCH_GEMM::runMI355XVariant(VariantID, size_t tune_idx) {
   const Index_type reps = getRunReps();
-  if (tune_idx == 2) {  // the slices' scratch is allocated outside the timed reps
-    const size_t bytes = rpx_gemm_mixed_scratch_bytes(m_ni, m_nj, m_nk, 7);
+  if (tune_idx == 2) {  // the residues. scratch is allocated outside the timed reps
+    const size_t bytes = rpx_gemm_mixed_scratch_bytes(m_ni, m_nj, m_nk, 0);
     char* scratch = nullptr;
     allocData(scratch, Size_type(bytes));
     timedReps(reps, [&](RepIndex_type) {
-      rpxCheck(rpx_gemm_mixed_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, 7, scratch, bytes, stream()),
+      rpxCheck(rpx_gemm_mixed_f64(m_C, m_A, m_B, m_alpha, m_beta, m_ni, m_nj, m_nk, 0, scratch, bytes, stream()),
                "GEMM mixed");
     });
     deallocData(scratch);

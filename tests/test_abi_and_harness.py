@@ -52,6 +52,35 @@ def test_scratch_queries():
     assert L.rpx_sort_segmented_scratch_bytes(offs, 3) >= 8 * 5000
 
 
+def test_mixed_gemm_bit_count_follows_the_moduli_product():
+    """b is the largest integer <= 52 with 2 nk 4^b < M, M the product of
+    the first T of the 16 largest pairwise coprime integers <= 256; the
+    default T is the smallest with b >= 42.  The moduli are recomputed here
+    from their definition."""
+    import math
+    mods = []
+    for m in range(256, 1, -1):
+        if all(math.gcd(m, x) == 1 for x in mods):
+            mods.append(m)
+        if len(mods) == 16:
+            break
+    L = abi.lib()
+    for nk in (1, 7, 64, 1000, 4096, 19020, 131071):
+        auto = None
+        for t in range(1, 17):
+            M = math.prod(mods[:t])
+            want = max((b for b in range(53) if 2 * nk * 4 ** b < M), default=-1)
+            got = L.rpx_gemm_mixed_bits(nk, t)
+            assert got == (want if want >= 1 else abi.RPX_ERR_UNSUPPORTED), (nk, t)
+            if auto is None and want >= 42:
+                auto = want
+        assert L.rpx_gemm_mixed_bits(nk, 0) == auto
+    assert L.rpx_gemm_mixed_bits(4096, 0) == 44 and L.rpx_gemm_mixed_bits(4096, 13) == 44
+    assert L.rpx_gemm_mixed_bits(131072, 0) == abi.RPX_ERR_INVALID_ARGUMENT
+    assert L.rpx_gemm_mixed_scratch_bytes(64, 64, 131072, 0) == 0
+    assert L.rpx_gemm_mixed_scratch_bytes(64, 64, 64, 17) == 0
+
+
 def run_harness(*args):
     return subprocess.run([HARNESS, *args], capture_output=True, text=True, timeout=120)
 

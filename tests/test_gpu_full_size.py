@@ -12,7 +12,7 @@
   path and against the oracle's Base_OpenMP.
 * MASS3DPA and DIFFUSION3DPA at NE = 262,144, (D1D, Q1D) = (4, 5), with the
   Gauss-Lobatto basis, bit-exact against the oracle's Base_OpenMP.
-* GEMM 4096^3 (alpha 1.5): FP64 MFMA and 7-slice mixed, on sampled rows
+* GEMM 4096^3 (alpha 1.5): FP64 MFMA and the 13-modulus mixed path, on sampled rows
   against 80-bit host arithmetic.
 """
 
@@ -228,10 +228,11 @@ def test_diffusion3dpa_config3_bit_exact(rpx, orc):
 
 
 def test_gemm_4096_sampled_rows(rpx):
-    """FP64 MFMA within its forward-error bound and the 7-slice mixed path
-    within (s+3) 2^-7s |alpha| 2^(e_i+f_j) nk, where 2^e_i and 2^f_j are the
-    power-of-two scales of row i of A and column j of B, both against 80-bit
-    host arithmetic on sampled rows (including the first and last)."""
+    """FP64 MFMA within its forward-error bound and the mixed path (13
+    moduli, b = 44 bits) within |alpha| 2^(e_i+f_j-b) nk (1 + 2^-b), where
+    2^e_i and 2^f_j are the power-of-two scales of row i of A and column j
+    of B, both against 80-bit host arithmetic on sampled rows (including the
+    first and last)."""
     L = rpx.lib()
     n, alpha = 4096, 1.5
     dA, dB = rpx.DeviceBuffer(8 * n * n), rpx.DeviceBuffer(8 * n * n)
@@ -239,9 +240,11 @@ def test_gemm_4096_sampled_rows(rpx):
     _fill(L, dB, n * n, 1, -1.0, 1.0)
     Cf, Cm = rpx.DeviceBuffer(8 * n * n), rpx.DeviceBuffer(8 * n * n)
     L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None)
-    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 7)
+    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 0)
     scr = rpx.DeviceBuffer(sb)
-    L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 7, scr.ptr, sb, None)
+    L.rpx_gemm_mixed_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, 0, scr.ptr, sb, None)
+    b = L.rpx_gemm_mixed_bits(n, 0)
+    assert b == 44
     A = dA.to_array().reshape(n, n)
     Bh = dB.to_array().reshape(n, n)
     rows = np.sort(np.random.default_rng(3).choice(n, 8, replace=False))
@@ -254,7 +257,7 @@ def test_gemm_4096_sampled_rows(rpx):
     cm = Cm.to_array().reshape(n, n)[rows]
     _, e = np.frexp(np.abs(A[rows]).max(axis=1, keepdims=True))
     _, f = np.frexp(np.abs(Bh).max(axis=0, keepdims=True))
-    bound = abs(alpha) * np.ldexp(1.0, e + f) * n * (7 + 3) * 2.0 ** (-49)
+    bound = abs(alpha) * np.ldexp(1.0, e + f - b) * n * (1 + 2.0 ** -b) + 4 * eps * np.abs(ref.astype(np.float64))
     assert np.all(np.abs(cm - ref) <= bound)
     # and the per-entry relative error the bench reports
     assert float(np.max(np.abs(cm - ref) / scale)) <= 1e-12

@@ -1,43 +1,68 @@
-"""GPU: mixed-precision GEMM and MASS3DPA (exact int8 slice products on
-MFMA, FP64 recombination).  GEMM error is measured per element against an
-80-bit (long double) reference, relative to |alpha| (|A||B|)_ij, the
-conditioning of each entry; MASS3DPA error against the FP64 oracle relative
-to the operator applied to absolute values.  The stated tolerance is 1e-12
-at the default 7 slices."""
+"""GPU: mixed-precision GEMM, MASS3DPA and DIFFUSION3DPA on int8 matrix
+cores with FP64 results.  GEMM (the modular scheme: one int8 GEMM per
+modulus, FP64 reconstruction) is measured per element against an 80-bit
+(long double) reference, both relative to |alpha| (|A||B|)_ij, the
+conditioning of each entry, and against the stated bound
+|alpha| 2^(e_i+f_j-b) nk (1 + 2^-b) from the row/column scales and the bit
+count b the library reports; MASS3DPA and DIFFUSION3DPA against the FP64
+oracle relative to the operator applied to absolute values.  The stated
+tolerance is 1e-12 at the default settings."""
 
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
+EPS = np.finfo(np.float64).eps
+
 
 def dev(rpx, a):
     return rpx.DeviceBuffer.from_array(np.ascontiguousarray(a, dtype=np.float64))
 
 
-def run_mixed(rpx, A, B, alpha, ni, nj, nk, s=0):
+def run_mixed(rpx, A, B, alpha, ni, nj, nk, t=0):
     L = rpx.lib()
-    sb = L.rpx_gemm_mixed_scratch_bytes(ni, nj, nk, s)
+    sb = L.rpx_gemm_mixed_scratch_bytes(ni, nj, nk, t)
+    assert sb > 0
     scr = rpx.DeviceBuffer(sb)
     dA, dB, dC = dev(rpx, A), dev(rpx, B), rpx.DeviceBuffer(8 * ni * nj)
-    L.rpx_gemm_mixed_f64(dC.ptr, dA.ptr, dB.ptr, alpha, 1.2, ni, nj, nk, s, scr.ptr, sb, None)
+    print(f"mixed GEMM {ni}x{nj}x{nk} moduli={t}", flush=True)
+    L.rpx_gemm_mixed_f64(dC.ptr, dA.ptr, dB.ptr, alpha, 1.2, ni, nj, nk, t, scr.ptr, sb, None)
     return dC.to_array().reshape(ni, nj)
 
 
+def exact(A, B, alpha, ni, nj, nk):
+    return (A.reshape(ni, nk).astype(np.longdouble) @ B.reshape(nk, nj).astype(np.longdouble)) * np.longdouble(alpha)
+
+
 def rel_err(C, A, B, alpha, ni, nj, nk):
-    Al = A.reshape(ni, nk).astype(np.longdouble)
-    Bl = B.reshape(nk, nj).astype(np.longdouble)
-    ref = (Al @ Bl) * np.longdouble(alpha)
+    ref = exact(A, B, alpha, ni, nj, nk)
     scale = abs(alpha) * (np.abs(A.reshape(ni, nk)) @ np.abs(B.reshape(nk, nj)))
     return np.max(np.abs(C.astype(np.longdouble) - ref) / np.maximum(scale, 1e-300))
 
 
-@pytest.mark.parametrize("ni,nj,nk", [(64, 64, 64), (130, 70, 200), (257, 129, 1000), (512, 512, 512)])
+def stated_bound(rpx, A, B, alpha, ni, nj, nk, t=0):
+    """|alpha| 2^(e_i+f_j-b) nk (1 + 2^-b), with 2^(e-1) <= max|row| < 2^e,
+    plus the FP64 rounding of the result."""
+    b = rpx.lib().rpx_gemm_mixed_bits(nk, t)
+    assert b > 0
+    _, e = np.frexp(np.abs(A.reshape(ni, nk)).max(axis=1))
+    _, f = np.frexp(np.abs(B.reshape(nk, nj)).max(axis=0))
+    scales = np.ldexp(1.0, np.add.outer(e, f) - b)
+    ref = exact(A, B, alpha, ni, nj, nk)
+    return abs(alpha) * scales * nk * (1 + 2.0 ** -b) + 4 * EPS * np.abs(ref.astype(np.float64)), b
+
+
+@pytest.mark.parametrize("ni,nj,nk", [(64, 64, 64), (130, 70, 200), (257, 129, 1000), (512, 512, 512),
+                                      (300, 520, 1)])
 def test_mixed_gemm_meets_1e12(rpx, rng, ni, nj, nk):
     A = rng.uniform(-1, 1, ni * nk)
     B = rng.uniform(-1, 1, nk * nj)
     C = run_mixed(rpx, A, B, 1.5, ni, nj, nk)
     assert rel_err(C, A, B, 1.5, ni, nj, nk) <= 1e-12
+    bound, b = stated_bound(rpx, A, B, 1.5, ni, nj, nk)
+    assert b >= 42
+    assert np.all(np.abs(C - exact(A, B, 1.5, ni, nj, nk)) <= bound)
 
 
 def test_mixed_gemm_wide_dynamic_range(rpx, rng):
@@ -45,49 +70,56 @@ def test_mixed_gemm_wide_dynamic_range(rpx, rng):
     A = rng.uniform(-1, 1, ni * nk) * np.exp2(rng.integers(-30, 30, ni * nk))
     B = rng.uniform(-1, 1, nk * nj) * np.exp2(rng.integers(-30, 30, nk * nj))
     A[:nk] = 0.0  # an all-zero row
+    B[5::nj] = 0.0  # an all-zero column
     C = run_mixed(rpx, A, B, -0.62, ni, nj, nk)
-    assert np.all(C[0] == 0.0)
-    # entries far below their row/column maxima carry less absolute precision;
-    # the bound is relative to max_k|A[i,k]| max_k|B[k,j]| * nk
-    rowmax = np.abs(A.reshape(ni, nk)).max(axis=1)
-    colmax = np.abs(B.reshape(nk, nj)).max(axis=0)
-    ref = (A.reshape(ni, nk).astype(np.longdouble) @ B.reshape(nk, nj).astype(np.longdouble)) * np.longdouble(-0.62)
-    bound = 0.62 * nk * np.outer(rowmax, colmax) * 1e-13
-    assert np.all(np.abs(C - ref) <= bound)
+    assert np.all(C[0] == 0.0) and np.all(C[:, 5] == 0.0)
+    # entries far below their row/column maxima carry less absolute precision:
+    # the bound is relative to the row and column scales, not to (|A||B|)_ij
+    bound, _ = stated_bound(rpx, A, B, -0.62, ni, nj, nk)
+    assert np.all(np.abs(C - exact(A, B, -0.62, ni, nj, nk)) <= bound)
 
 
-def test_mixed_gemm_slices_trade_accuracy(rpx, rng):
+def test_mixed_gemm_moduli_trade_accuracy(rpx, rng):
     ni = nj = nk = 128
     A = rng.uniform(-1, 1, ni * nk)
     B = rng.uniform(-1, 1, nk * nj)
-    errs = [rel_err(run_mixed(rpx, A, B, 1.0, ni, nj, nk, s), A, B, 1.0, ni, nj, nk) for s in (2, 3, 4, 5, 6, 7, 8)]
-    assert all(errs[i + 1] < errs[i] for i in range(4))  # each slice gains ~7 bits until FP64 rounding
-    for s, e in zip((2, 3, 4, 5, 6, 7, 8), errs):
-        assert e <= 4 * (s + 3) * 2.0 ** (-7 * s) + 1e-15
+    ts = list(range(2, 17))
+    bits = [rpx.lib().rpx_gemm_mixed_bits(nk, t) for t in ts]
+    assert all(bits[i + 1] >= bits[i] for i in range(len(bits) - 1)) and bits[-1] == 52
+    errs = []
+    for t in ts:
+        C = run_mixed(rpx, A, B, 1.0, ni, nj, nk, t)
+        bound, _ = stated_bound(rpx, A, B, 1.0, ni, nj, nk, t)
+        assert np.all(np.abs(C - exact(A, B, 1.0, ni, nj, nk)) <= bound), t
+        errs.append(rel_err(C, A, B, 1.0, ni, nj, nk))
+    # each modulus adds about 4 bits to A and to B until FP64 rounding
+    assert all(errs[i + 1] < errs[i] for i in range(6))
+    assert errs[-1] <= 1e-15
 
 
-@pytest.mark.parametrize("s,nk_max", [(8, 16643), (7, 19020)])
-def test_mixed_gemm_nk_limit_all_digits_maximal(rpx, s, nk_max):
-    """The first seven base-128 digits of 1 - 2^-50 are 127 (the eighth is
-    64), so at s = 7 the top level of the int32 accumulation reaches
-    7 * 127^2 * nk: at the largest accepted nk it stays below 2^31 and the
-    result matches an 80-bit sum.  The limit is s * 127^2 * nk < 2^31 for
-    every s, and one more nk is refused."""
+def test_mixed_gemm_nk_limit_with_extreme_residues(rpx):
+    """A = B = 1 - 2^(7-b) makes every scaled entry 2^b - 128, whose residue
+    modulo 256 is -128 (for B, times an odd y_0: still -128), so the first
+    modulus's int32 sums reach 128 * 128 * nk.  At the largest accepted nk,
+    131071, that is 2^31 - 16384; the result matches the 80-bit product, and
+    one more nk is refused."""
     L = rpx.lib()
     ni = nj = 16
-    v = 1.0 - 2.0 ** -50
-    assert s * 127 ** 2 * nk_max < 2 ** 31 <= s * 127 ** 2 * (nk_max + 1)
-    A = np.full(ni * nk_max, v)
-    B = np.full(nk_max * nj, v)
-    C = run_mixed(rpx, A, B, 1.0, ni, nj, nk_max, s)
-    ref = np.longdouble(nk_max) * np.longdouble(v) * np.longdouble(v)
-    assert np.all(np.abs(C - ref) <= (s + 3) * 2.0 ** (-7 * s) * nk_max + 4 * nk_max * 2.0 ** -53)
-    sb = L.rpx_gemm_mixed_scratch_bytes(ni, nj, nk_max + 1, s)
-    scr = rpx.DeviceBuffer(sb)
-    dA, dB, dC = rpx.DeviceBuffer(8 * ni * (nk_max + 1)), rpx.DeviceBuffer(8 * nj * (nk_max + 1)), \
+    nk = 131071
+    assert 128 * 128 * nk < 2 ** 31 <= 128 * 128 * (nk + 1)
+    b = L.rpx_gemm_mixed_bits(nk, 0)
+    v = 1.0 - 2.0 ** (7 - b)
+    A = np.full(ni * nk, v)
+    B = np.full(nk * nj, v)
+    C = run_mixed(rpx, A, B, 1.0, ni, nj, nk)
+    ref = np.longdouble(nk) * np.longdouble(v) * np.longdouble(v)
+    assert np.all(np.abs(C - ref) <= 4 * EPS * float(ref))
+    assert L.rpx_gemm_mixed_scratch_bytes(ni, nj, nk + 1, 0) == 0
+    dA, dB, dC = rpx.DeviceBuffer(8 * ni * (nk + 1)), rpx.DeviceBuffer(8 * nj * (nk + 1)), \
         rpx.DeviceBuffer(8 * ni * nj)
+    scr = rpx.DeviceBuffer(1 << 20)
     with pytest.raises(rpx.RpxError) as ei:
-        L.rpx_gemm_mixed_f64(dC.ptr, dA.ptr, dB.ptr, 1.0, 0.0, ni, nj, nk_max + 1, s, scr.ptr, sb, None)
+        L.rpx_gemm_mixed_f64(dC.ptr, dA.ptr, dB.ptr, 1.0, 0.0, ni, nj, nk + 1, 0, scr.ptr, 1 << 20, None)
     assert ei.value.status == rpx.RPX_ERR_UNSUPPORTED
 
 
