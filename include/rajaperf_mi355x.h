@@ -171,6 +171,22 @@ int rpx_reduce_scratch_init(void* scratch, size_t scratch_bytes,
 int rpx_dot_f64(const double* a, const double* b, int64_t n, double* result,
                 int accumulate, void* scratch, size_t scratch_bytes,
                 const rpx_tuning* tune, void* stream);
+/* Allocation-free form of rpx_dot_f64, with the same kernels and results
+ * (bit-identical for a given (n, tune)).  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The scratch is a library-owned device array of
+ * rpx_dot_noalloc_max_scratch_bytes() bytes (1 MiB) per GPU, held until the
+ * process ends; shapes whose rpx_reduce_scratch_bytes exceeds it (beyond
+ * about 1.2e9 doubles at the default tuning) return
+ * RPX_ERR_SCRATCH_TOO_SMALL.  Every call on a GPU shares that GPU's array,
+ * so calls that overlap on one GPU (other streams, other host threads)
+ * race; calls ordered on one stream do not.  The first call on a device
+ * looks the array up and zeroes it on `stream`. */
+size_t rpx_dot_noalloc_max_scratch_bytes(void);
+int rpx_dot_f64_noalloc(const double* a, const double* b, int64_t n,
+                        double* result, int accumulate,
+                        const rpx_tuning* tune, void* stream);
 /* *result = sum x[i]  (src/algorithm/REDUCE_SUM.hpp:31-32,
  * REDUCE_SUM-Seq.cpp:34-51, REDUCE_SUM-Hip.cpp:36-198). */
 int rpx_reduce_sum_f64(const double* x, int64_t n, double* result,

@@ -96,6 +96,8 @@ _SIGS = {
     "rpx_reduce_scratch_init": (_i32, [_vp, _sz, _vp]),
     "rpx_dot_f64": (_i32, [_vp, _vp, _i64, _vp, _i32, _vp, _sz, _tp, _vp]),
     "rpx_reduce_sum_f64": (_i32, [_vp, _i64, _vp, _vp, _sz, _tp, _vp]),
+    "rpx_dot_noalloc_max_scratch_bytes": (_sz, []),
+    "rpx_dot_f64_noalloc": (_i32, [_vp, _vp, _i64, _vp, _i32, _tp, _vp]),
     "rpx_scan_scratch_bytes": (_sz, [_i64, _tp]),
     "rpx_scan_scratch_init": (_i32, [_vp, _sz, _vp]),
     "rpx_scan_scratch_status": (_i32, [_vp]),

@@ -24,7 +24,16 @@
 // were partials) start at zero.
 #include "rpx_common.hpp"
 
+#include <mutex>
+
 using namespace rpx;
+
+// Library-owned scratch of the allocation-free DOT entry point
+// (rpx_dot_f64_noalloc).  Each device holds its own copy of this array from
+// module load until the process ends; host code reaches a device's copy only
+// through the address hipGetSymbolAddress returns for it.  1 MiB holds the
+// reduction layout of the default tuning up to about 1.2e9 doubles.
+static __device__ __align__(256) unsigned char scratch[1u << 20];
 
 namespace {
 
@@ -282,6 +291,33 @@ int run_reduce(const double* a, const double* b, int64_t n, double* result, int 
   return launch_status();
 }
 
+// Per-device address of `scratch`, looked up on a device's first
+// allocation-free call with that device current.  The tickets in the array
+// are zeroed then, on the caller's stream; every complete call leaves them
+// zero again.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(hipStream_t s, void** out) {
+  int dev = -1;
+  RPX_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return RPX_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    RPX_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    const int64_t words = int64_t(sizeof(scratch) / sizeof(unsigned));
+    zero_u32_kernel<<<dim3(unsigned(ceil_div(words, 256))), dim3(256), 0, s>>>(reinterpret_cast<unsigned*>(p),
+                                                                                 words);
+    const int st = launch_status();
+    if (st != RPX_SUCCESS) return st;
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return RPX_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -308,6 +344,20 @@ int rpx_reduce_scratch_init(void* scratch, size_t scratch_bytes, void* stream) {
 int rpx_dot_f64(const double* a, const double* b, int64_t n, double* result, int accumulate, void* scratch,
                 size_t scratch_bytes, const rpx_tuning* tune, void* stream) {
   return run_reduce<true>(a, b, n, result, accumulate, scratch, scratch_bytes, tune, stream);
+}
+
+size_t rpx_dot_noalloc_max_scratch_bytes(void) { return sizeof(scratch); }
+
+int rpx_dot_f64_noalloc(const double* a, const double* b, int64_t n, double* result, int accumulate,
+                        const rpx_tuning* tune, void* stream) {
+  if (n < 0 || !result || (n > 0 && (!a || !b))) return RPX_ERR_INVALID_ARGUMENT;
+  const size_t need = rpx_reduce_scratch_bytes(n, tune);
+  if (need == 0) return RPX_ERR_UNSUPPORTED;
+  if (need > sizeof(scratch)) return RPX_ERR_SCRATCH_TOO_SMALL;
+  void* p = nullptr;
+  const int st = noalloc_scratch(as_stream(stream), &p);
+  if (st != RPX_SUCCESS) return st;
+  return run_reduce<true>(a, b, n, result, accumulate, p, sizeof(scratch), tune, stream);
 }
 
 int rpx_reduce_sum_f64(const double* x, int64_t n, double* result, void* scratch, size_t scratch_bytes,
