@@ -296,7 +296,7 @@ int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha,
  * 2^f_j for B) and rounded to b-bit integers; their exact integer product
  * is assembled from one int8 GEMM per modulus (`nmoduli` of the 16 largest
  * pairwise coprime integers <= 256) and converted to FP64.  b is the
- * largest value with 2 nk 4^b below the moduli's product (at most 52;
+ * largest value with 2 nk 4^b below the moduli's product (at most 46;
  * rpx_gemm_mixed_bits returns it), and the only error besides the FP64
  * rounding of C is that rounding of A and B:
  *   |C[i][j] - alpha (AB)[i][j]| <= |alpha| 2^(e_i+f_j-b) nk (1 + 2^-b),

@@ -10,24 +10,27 @@
 //     to integers: |A'|, |B'| <= 2^b.  X = A'B' is an exact integer with
 //     |X| <= nk 4^b.
 //   * T pairwise coprime moduli m_t <= 256 (the largest such numbers) with
-//     M = prod m_t > 2 nk 4^b.  For each t the residues of A' and of
-//     y_t B' (y_t = (M/m_t)^-1 mod m_t) lie in [-128, 127] and one GEMM on
+//     M = prod m_t > 2 nk 4^b.  For each t the symmetric residues of A' and
+//     B' lie in [-128, 128] (128 only for m_t = 256, where it is stored as
+//     the byte -128, the same residue) and one GEMM on
 //     v_mfma_i32_16x16x64_i8 gives their product exactly in int32 (nk <=
-//     131071); its epilogue keeps d_t = that product mod m_t = y_t X mod m_t,
-//     one byte per output.
-//   * Chinese remainder theorem: X = sum_t d_t M/m_t (mod M), so
-//     X/M = sum_t d_t/m_t minus the nearest integer, |X/M| < 1/2.  With
-//     1/m_t = h_t + l_t, h_t a multiple of 2^-52, the sum of d_t h_t is kept
-//     exactly in FP64 modulo 1 and the sum of d_t l_t (below T 2^-44) is
-//     added after the integer part is removed, so X comes out with a
-//     relative error of a few 2^-53 however much it cancels.
+//     131071); its epilogue keeps d_t = that product modulo m_t, a
+//     symmetric residue of X, one byte per output.
+//   * Chinese remainder theorem: X = sum_t d_t y_t M/m_t (mod M), y_t =
+//     (M/m_t)^-1 mod m_t, so X/M = sum_t d_t y_t/m_t minus the nearest
+//     integer, |X/M| < 1/2.  With y_t/m_t = h_t + l_t, h_t a multiple of
+//     2^-44, the sum of d_t h_t is kept exactly in FP64 modulo 1 and the sum
+//     of d_t l_t (below T 2^-36) is added after the integer part is removed,
+//     so X comes out with a relative error of a few 2^-53, plus an absolute
+//     error below 2^-84 M, however much it cancels.
 //   * C[i][j] = alpha 2^(e_i+f_j-2b) X.
 // The only approximation is the rounding of A and B to b bits below their
 // row/column maxima: |C - alpha AB|_ij <= |alpha| 2^(e_i+f_j-b) nk (1+2^-b)
 // plus FP64 rounding of the result.  b is the largest value with
-// 2 nk 4^b < M (at most 52); by default T is the smallest count giving
-// b >= 42 (13 moduli, b = 44 at nk = 4096), an error below 2^-42 relative
-// to |alpha| max_k|A[i][k]| max_k|B[k][j]| nk.
+// 2 nk 4^b < M, at most 46 (so every quotient below is exact, see
+// sym_mod); by default T is the smallest count giving b >= 42 (13 moduli,
+// b = 44 at nk = 4096), an error below 2^-42 relative to
+// |alpha| max_k|A[i][k]| max_k|B[k][j]| nk.
 //
 // Scratch holds the int8 residues (A: [t][i][k], B: [t][j][k], K
 // contiguous, zero-padded to tile multiples), the per-row/column exponents
@@ -44,6 +47,7 @@ constexpr int kMaxMods = 16;
 // The 16 largest pairwise coprime integers not above 256.
 constexpr int kMods[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
 constexpr int kMinBits = 42;       // automatic modulus count: the smallest T with b >= 42
+constexpr int kMaxBits = 46;       // |A'|, |B'| <= 2^46: exact quotients in sym_mod
 constexpr int64_t kMaxK = 131071;  // 128 * 128 * nk < 2^31
 
 constexpr int kTM = 256, kTN = 256, kTK = 128;  // block tile, K bytes per stage
@@ -57,10 +61,8 @@ struct CrtParams {
   int T, b;
   double m[kMaxMods];    // moduli
   double inv[kMaxMods];  // fl(1/m_t), used only to pick a quotient that is then corrected
-  double lo[kMaxMods];   // smallest symmetric residue, -floor(m_t/2)
-  double y[kMaxMods];    // (M/m_t)^-1 mod m_t
-  double h[kMaxMods];    // floor(2^52/m_t) 2^-52
-  double l[kMaxMods];    // 1/m_t - h_t, rounded
+  double h[kMaxMods];    // floor(2^44 y_t/m_t) 2^-44, y_t = (M/m_t)^-1 mod m_t
+  double l[kMaxMods];    // y_t/m_t - h_t, rounded
   double M_hi, M_lo;     // prod m_t = M_hi + M_lo, M_lo rounded (a 2^-106 relative error)
 };
 
@@ -76,12 +78,12 @@ int64_t inverse_mod(int64_t a, int64_t m) {
   return t < 0 ? t + m : t;
 }
 
-// Largest b <= 52 with 2 nk 4^b < M, or -1.
+// Largest b <= kMaxBits with 2 nk 4^b < M, or -1.
 int bits_for(int64_t nk, int T) {
   u128 M = 1;
   for (int t = 0; t < T; ++t) M *= u128(kMods[t]);
   const u128 k = u128(nk > 0 ? nk : 1);
-  for (int b = 52; b >= 0; --b)
+  for (int b = kMaxBits; b >= 0; --b)
     if (2 * k * (u128(1) << (2 * b)) < M) return b;
   return -1;
 }
@@ -113,14 +115,13 @@ bool crt_setup(int64_t nk, int nmoduli, CrtParams* p) {
     const int64_t m = kMods[t];
     p->m[t] = double(m);
     p->inv[t] = 1.0 / double(m);
-    p->lo[t] = -double(m / 2);
     int64_t mt = 1;  // (M / m_t) mod m_t
     for (int s = 0; s < T; ++s)
       if (s != t) mt = mt * kMods[s] % m;
-    p->y[t] = t < T ? double(inverse_mod(mt, m)) : 0.0;
-    const uint64_t two52 = uint64_t(1) << 52;
-    p->h[t] = std::ldexp(double(two52 / uint64_t(m)), -52);
-    p->l[t] = std::ldexp(double(two52 % uint64_t(m)) / double(m), -52);
+    const uint64_t y = t < T ? uint64_t(inverse_mod(mt, m)) : 0;
+    const uint64_t num = y << 44;  // y < 256: below 2^52
+    p->h[t] = std::ldexp(double(num / uint64_t(m)), -44);
+    p->l[t] = std::ldexp(double(num % uint64_t(m)) / double(m), -44);
   }
   return true;
 }
@@ -149,14 +150,14 @@ __device__ __forceinline__ int exponent_of(double m) {
   return e;
 }
 
-// Symmetric residue of the integer-valued x (|x| <= 2^52) modulo m: the
-// quotient from fl(1/m) is within one of the true one, the fma is exact.
-__device__ __forceinline__ double sym_mod(double x, double m, double inv, double lo) {
-  const double q = rint(x * inv);
-  double r = fma(-q, m, x);
-  if (r < lo) r += m;
-  if (r >= lo + m) r -= m;
-  return r;
+// Symmetric residue of the integer-valued x modulo m, |x| <= 2^46 (or
+// |x| < 2^31): fl(x fl(1/m)) is within 2^-13 of x/m, and x/m is at least
+// 1/(2m) >= 2^-9 from a half-integer unless m = 256 and x = 128 (mod 256).
+// So q = rint(x/m) exactly, the fma is exact, and the result lies in
+// [-(m-1)/2, (m-1)/2], or in [-128, 128] for m = 256, where the byte kept
+// (r & 0xFF) makes 128 the same residue as -128.
+__device__ __forceinline__ int sym_mod(double x, double m, double inv) {
+  return int(fma(-rint(x * inv), m, x));
 }
 
 // One workgroup per (padded) row of A: row exponent, then the residues.
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
         int packed = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = int(sym_mod(x[4 * c + q], P.m[t], P.inv[t], P.lo[t]));
+          const int r = sym_mod(x[4 * c + q], P.m[t], P.inv[t]);
           packed |= (r & 0xFF) << (8 * q);
         }
         w[c] = packed;
@@ -243,44 +244,42 @@ __global__ void fill_int_kernel(int* p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
-// B -> Bs[t][j][k] through an LDS transpose: a workgroup owns 64 columns x
-// 64 k and writes one modulus at a time.  The residues are those of
-// y_t B', so the GEMM's product is y_t X modulo m_t.
-__global__ __launch_bounds__(256) void split_cols_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
+// B -> Bs[t][j][k]: a workgroup owns 64 columns x 128 k; work-item (w, j)
+// reads k0 + 16w .. k0 + 16w + 15 of column j (each load is 64 consecutive
+// columns of one row) and writes those 16 residues as one 16-byte chunk per
+// modulus, so no transpose through LDS is needed, and the workgroup's eight
+// chunks of a row make one whole 128-byte line.
+__global__ __launch_bounds__(512) void split_cols_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
                                                          int64_t nkp, int64_t njp, const int* __restrict__ fb,
                                                          signed char* __restrict__ Bs, const CrtParams P) {
-  __shared__ __attribute__((aligned(16))) signed char lds[64][64 + 16];
-  const int64_t j0 = int64_t(blockIdx.x) * 64, k0 = int64_t(blockIdx.y) * 64;
   const int tid = threadIdx.x;
-  const int jj = tid & 63;
-  const int64_t j = j0 + jj;
+  const int64_t j = int64_t(blockIdx.x) * 64 + (tid & 63);
+  const int64_t k0 = int64_t(blockIdx.y) * 128 + 16 * (tid >> 6);
   const int f = (j < nj) ? fb[j] : 0;
   const int sh = P.b - (f == INT32_MIN ? 0 : f);
   double x[16];
-  if (j0 + 64 <= nj && k0 + 64 <= nk) {
+  if (j < nj && k0 + 16 <= nk) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) x[u] = B[(k0 + (tid >> 6) + 4 * u) * nj + j];
+    for (int u = 0; u < 16; ++u) x[u] = B[(k0 + u) * nj + j];
   } else {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int64_t k = k0 + (tid >> 6) + 4 * u;
-      x[u] = (j < nj && k < nk) ? B[k * nj + j] : 0.0;
-    }
+    for (int u = 0; u < 16; ++u) x[u] = (j < nj && k0 + u < nk) ? B[(k0 + u) * nj + j] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < 16; ++u) x[u] = rint(ldexp(x[u], sh));
-  // 64 columns x 64 bytes per modulus: 16 bytes per work-item.
-  const int row = tid >> 2, part = (tid & 3) * 16;
   for (int t = 0; t < P.T; ++t) {
+    v4i w;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const double r = sym_mod(x[u], P.m[t], P.inv[t], P.lo[t]);
-      lds[jj][(tid >> 6) + 4 * u] = (signed char)int(sym_mod(r * P.y[t], P.m[t], P.inv[t], P.lo[t]));
+    for (int c = 0; c < 4; ++c) {
+      int packed = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = sym_mod(x[4 * c + q], P.m[t], P.inv[t]);
+        packed |= (r & 0xFF) << (8 * q);
+      }
+      w[c] = packed;
     }
-    __syncthreads();
-    const v4i w = *reinterpret_cast<const v4i*>(&lds[row][part]);
-    *reinterpret_cast<v4i*>(Bs + (int64_t(t) * njp + j0 + row) * nkp + k0 + part) = w;
-    __syncthreads();
+    *reinterpret_cast<v4i*>(Bs + (int64_t(t) * njp + j) * nkp + k0) = w;
   }
 }
 
@@ -387,9 +386,8 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
         __syncthreads();
       }
     }
-    // d_t = sum mod m_t in [0, m_t): |sum| < 2^31, so the fma is exact and
-    // the quotient from fl(1/m_t) is off by at most one.  The four rows of
-    // a lane's accumulator go to one 32-bit word.
+    // d_t = the sum's symmetric residue modulo m_t (|sum| < 2^31, see
+    // sym_mod); the four rows of a lane's accumulator go to one 32-bit word.
     const double m = P.m[t], inv = P.inv[t];
     unsigned* rt = R + int64_t(t) * per_mod + frag0 * kWave + lane;
 #pragma unroll
@@ -398,13 +396,7 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
       for (int j = 0; j < kFn; ++j) {
         unsigned w = 0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double c = double(acc[i][j][r]);
-          double d = fma(-floor(c * inv), m, c);
-          if (d < 0.0) d += m;
-          if (d >= m) d -= m;
-          w |= unsigned(int(d)) << (8 * r);
-        }
+        for (int r = 0; r < 4; ++r) w |= unsigned(sym_mod(double(acc[i][j][r]), m, inv) & 0xFF) << (8 * r);
         rt[(i * kFn + j) * kWave] = w;
       }
   }
@@ -412,8 +404,8 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
 
 // One work-item per (fragment, lane): the four outputs of that accumulator
 // slot.  X/M = sum_t d_t (h_t + l_t) mod 1; the h_t part is exact (every
-// term is a multiple of 2^-52 below 1 and the sum is reduced modulo 1 after
-// each add).
+// term is a multiple of 2^-44 below 128 in magnitude and the sum is reduced
+// modulo 1 after each add).
 __global__ __launch_bounds__(256) void crt_combine_kernel(double* __restrict__ C, const unsigned* __restrict__ R,
                                                           const int* __restrict__ ea, const int* __restrict__ fb,
                                                           double alpha, int64_t ni, int64_t nj, int64_t nip,
@@ -428,14 +420,19 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(double* __restrict__ C
   const int wave = f / (kFm * kFn), i = (f / kFn) % kFm, j = f % kFn;
   const int64_t row0 = (tile / tiles_n) * kTM + (wave >> 1) * kWaveM + i * 16 + 4 * (lane >> 4);
   const int64_t col = (tile % tiles_n) * kTN + (wave & 1) * kWaveN + j * 16 + (lane & 15);
+  // All residue words first (T <= 16), so their loads are in flight together.
+  unsigned wv[kMaxMods];
+#pragma unroll
+  for (int t = 0; t < kMaxMods; ++t) wv[t] = t < P.T ? R[int64_t(t) * per_mod + g] : 0u;
   double fh[4] = {0.0, 0.0, 0.0, 0.0}, fl[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int t = 0; t < P.T; ++t) {
-    const unsigned w = R[int64_t(t) * per_mod + g];
+#pragma unroll
+  for (int t = 0; t < kMaxMods; ++t) {
+    if (t >= P.T) break;
     const double h = P.h[t], l = P.l[t];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double d = double((w >> (8 * r)) & 0xFFu);
-      const double s = fma(d, h, fh[r]);  // exact: both terms are multiples of 2^-52 below 1
+      const double d = double(int(wv[t] << (24 - 8 * r)) >> 24);  // the signed byte r
+      const double s = fma(d, h, fh[r]);  // exact: multiples of 2^-44 below 129 in magnitude
       fh[r] = s - floor(s);
       fl[r] = fma(d, l, fl[r]);
     }
@@ -504,7 +501,7 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
   if (nk > 0)
     col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 256)), unsigned(ceil_div(nk, kchunk))), dim3(256), 0, s>>>(
         B, nk, nj, kchunk, fb);
-  split_cols_kernel<<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 64)), dim3(256), 0, s>>>(B, nk, nj, l.nkp, l.njp,
+  split_cols_kernel<<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 128)), dim3(512), 0, s>>>(B, nk, nj, l.nkp, l.njp,
                                                                                            fb, Bs, P);
   ozaki_gemm_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(R, As, Bs, l.nip, l.njp, l.nkp, int(tn), P);
   crt_combine_kernel<<<dim3(unsigned(ceil_div(l.nip * l.njp / 4, 256))), dim3(256), 0, s>>>(

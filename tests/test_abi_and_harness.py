@@ -53,7 +53,7 @@ def test_scratch_queries():
 
 
 def test_mixed_gemm_bit_count_follows_the_moduli_product():
-    """b is the largest integer <= 52 with 2 nk 4^b < M, M the product of
+    """b is the largest integer <= 46 with 2 nk 4^b < M, M the product of
     the first T of the 16 largest pairwise coprime integers <= 256; the
     default T is the smallest with b >= 42.  The moduli are recomputed here
     from their definition."""
@@ -69,7 +69,7 @@ def test_mixed_gemm_bit_count_follows_the_moduli_product():
         auto = None
         for t in range(1, 17):
             M = math.prod(mods[:t])
-            want = max((b for b in range(53) if 2 * nk * 4 ** b < M), default=-1)
+            want = max((b for b in range(47) if 2 * nk * 4 ** b < M), default=-1)
             got = L.rpx_gemm_mixed_bits(nk, t)
             assert got == (want if want >= 1 else abi.RPX_ERR_UNSUPPORTED), (nk, t)
             if auto is None and want >= 42:
