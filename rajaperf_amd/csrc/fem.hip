@@ -83,7 +83,7 @@ struct MassShape {
 // compiler may reorder accesses between them.  Ends with the workgroup's
 // last LDS reads still pending: a caller that reuses the buffers must
 // synchronise first.
-template <int D, int Q>
+template <int D, int Q, bool kEarlyXY = true>
 __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcdouble* __restrict__ Btg,
                                                   gcdouble* __restrict__ Dg, gcdouble* __restrict__ Xg,
                                                   gdouble* __restrict__ Yg, int64_t e0, int ne,
@@ -102,6 +102,26 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
   constexpr int kIt3 = (S::E * Q2 + kBS - 1) / kBS;  // P3 pencils per work-item
   constexpr bool kPre = kIt3 * Q <= kMassPrefetch;
   double dpre[kPre ? kIt3 : 1][Q];
+  // X (P1) and Y (P6) pencils go to registers early where they fit in eight
+  // doubles: X before the first barrier, Y at the start of P4 (when D's
+  // registers are free), so neither load waits at the head of its stage.
+  // Not at Q = D + 2 from D = 5, and not in the persistent loops (the
+  // single-launch batch kernel and the per-order loop), where the extra live
+  // registers cost a wavefront per SIMD.
+  constexpr int kIt1 = (S::E * D2 + kBS - 1) / kBS;  // P1 and P6 pencils per work-item
+  constexpr bool kPreXY = kEarlyXY && kIt1 * D <= 8 && !(D >= 5 && Q >= D + 2);
+  double xpre[kPreXY ? kIt1 : 1][D];
+  if constexpr (kPreXY) {
+#pragma unroll
+    for (int it = 0; it < kIt1; ++it) {
+      const int w = tid + it * kBS;
+      if (w < ne * D2) {
+        gcdouble* xp = Xg + (e0 + w / D2) * D3 + (w % D2) * D;  // X[e][dz][dy][:]
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) xpre[it][dx] = xp[dx];
+      }
+    }
+  }
   if constexpr (kPre) {
 #pragma unroll
     for (int it = 0; it < kIt3; ++it) {
@@ -116,13 +136,18 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
   __syncthreads();
 
   // P1: x-contraction of X pencils.
-  for (int w = tid; w < ne * D2; w += kBS) {
+  for (int w = tid, it = 0; w < ne * D2; w += kBS, ++it) {
     if constexpr (D >= 7) reload_basis();
     const int e = w / D2, r = w % D2, dz = r / D, dy = r % D;
-    gcdouble* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
     double x[D];
+    if constexpr (kPreXY) {
 #pragma unroll
-    for (int dx = 0; dx < D; ++dx) x[dx] = xp[dx];
+      for (int dx = 0; dx < D; ++dx) x[dx] = xpre[it][dx];
+    } else {
+      gcdouble* xp = Xg + (e0 + e) * D3 + dz * D2 + dy * D;
+#pragma unroll
+      for (int dx = 0; dx < D; ++dx) x[dx] = xp[dx];
+    }
     double* out = bufA + e * S::kA + (dz * D + dy) * Q;
 #pragma unroll
     for (int qx = 0; qx < Q; ++qx) {
@@ -178,6 +203,18 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
     }
   }
   __syncthreads();
+  double ypre[kPreXY ? kIt1 : 1][D];
+  if constexpr (kPreXY) {
+#pragma unroll
+    for (int it = 0; it < kIt1; ++it) {
+      const int w = tid + it * kBS;
+      if (w < ne * D2) {
+        gcdouble* yp = Yg + (e0 + w / D2) * D3 + w % D2;  // Y[e][:][dy][dx]
+#pragma unroll
+        for (int dz = 0; dz < D; ++dz) ypre[it][dz] = yp[dz * D2];
+      }
+    }
+  }
   // P4: backward x-contraction.
   for (int w = tid; w < ne * Q2; w += kBS) {
     if constexpr (D >= 7) reload_basis();
@@ -215,7 +252,7 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
   }
   __syncthreads();
   // P6: backward z-contraction, accumulate into Y.
-  for (int w = tid; w < ne * D2; w += kBS) {
+  for (int w = tid, it = 0; w < ne * D2; w += kBS, ++it) {
     if constexpr (D >= 7) reload_basis();
     const int e = w / D2, r = w % D2;  // r = dy*D + dx
     const double* in = bufA + e * S::kA + r;  // QDD[qz][dy][dx], stride D2 in qz
@@ -227,8 +264,13 @@ __device__ __forceinline__ void mass3dpa_elements(gcdouble* __restrict__ Bg, gcd
     // the Y pointer comes from memory the compiler otherwise keeps each
     // load behind the previous store and waits on it D times per pencil.
     double y[D];
+    if constexpr (kPreXY) {
 #pragma unroll
-    for (int dz = 0; dz < D; ++dz) y[dz] = yp[dz * D2];
+      for (int dz = 0; dz < D; ++dz) y[dz] = ypre[it][dz];
+    } else {
+#pragma unroll
+      for (int dz = 0; dz < D; ++dz) y[dz] = yp[dz * D2];
+    }
 #pragma unroll
     for (int dz = 0; dz < D; ++dz) {
       double u = 0.0;
@@ -285,7 +327,7 @@ __device__ __attribute__((noinline)) void mass_batch_chunk(rpx_mass_batch b, int
   double* sB = bufB + S::E * S::kB;
   // The descriptors' pointers are global memory: typed so, the element code
   // issues global (not FLAT) accesses, which do not hold up its LDS waits.
-  mass3dpa_elements<D, D + 1>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0, ne,
+  mass3dpa_elements<D, D + 1, false>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0, ne,
                               bufA, bufB, sB, sB + (D + 1) * D);
 }
 
@@ -394,7 +436,7 @@ __global__ __launch_bounds__(kBS) void mass3dpa_order_kernel(const char* __restr
     }
     const int64_t e0 = (c - cb) * S::E;
     const int ne = int((b.ne - e0) < S::E ? (b.ne - e0) : S::E);
-    mass3dpa_elements<D, D + 1>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0,
+    mass3dpa_elements<D, D + 1, false>((gcdouble*)b.B, (gcdouble*)b.Bt, (gcdouble*)b.D, (gcdouble*)b.X, (gdouble*)b.Y, e0,
                                 ne, bufA, bufB, sB, sBt);
     __syncthreads();  // the next chunk rewrites the LDS
   }

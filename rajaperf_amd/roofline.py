@@ -216,6 +216,42 @@ def config4(rows, reps, sample_rows=8):
         del scr
     del dA, dB, Cf, Cm
     mass_mixed(rows, reps)
+    diffusion_mixed(rows, reps)
+
+
+def diffusion_mixed(rows, reps):
+    """DIFFUSION3DPA of config 3 (symmetric-tensor D) with its contractions on
+    int8 MFMA, against the FP64 kernel.  Error: the largest deviation from
+    the FP64 kernel's result relative to the largest |Y| (the per-output
+    magnitude-bound check is in the GPU tests)."""
+    L = abi.lib()
+    NE, D, Q = 262144, 4, 5
+    B = abi.DeviceBuffer
+    Bm, G = B(8 * Q * D), B(8 * Q * D)
+    L.rpx_fill_uniform_f64(Bm.ptr, Q * D, SEED, 0, -1.0, 1.0, None)
+    L.rpx_fill_uniform_f64(G.ptr, Q * D, SEED, 1, -1.0, 1.0, None)
+    Dd = B(8 * NE * 6 * Q ** 3)
+    L.rpx_fill_sym_tensor_f64(Dd.ptr, NE, Q ** 3, 6, SEED, 2, None)
+    X = B(8 * NE * D ** 3)
+    L.rpx_fill_uniform_f64(X.ptr, NE * D ** 3, SEED, 3, -1.0, 1.0, None)
+    Yf, Ym = B(8 * NE * D ** 3), B(8 * NE * D ** 3)
+    nbytes = 2 * 8 * Q * D + 2 * 8 * D ** 3 * NE + 6 * 8 * Q ** 3 * NE + 8 * D ** 3 * NE
+    flops = NE * (Q * D + 5 * D * D * Q * D + 7 * D * D * Q * Q + 7 * Q * D * Q * Q + 15 * Q ** 3 + Q * D
+                  + 7 * Q * Q * D * Q + 7 * Q * Q * D * D + 7 * D * Q * D * D + 3 * D ** 3)
+    sb = L.rpx_diffusion3dpa_mixed_scratch_bytes(D, Q, 7)
+    scr = B(sb)
+    ms_f64 = _time(lambda: L.rpx_diffusion3dpa_f64(Bm.ptr, G.ptr, Dd.ptr, X.ptr, Yf.ptr, NE, D, Q, 1, None, None),
+                   reps)
+    ms_mix = _time(lambda: L.rpx_diffusion3dpa_mixed_f64(Bm.ptr, G.ptr, Dd.ptr, X.ptr, Ym.ptr, NE, D, Q, 1, 7,
+                                                         scr.ptr, sb, None), reps)
+    for y in (Yf, Ym):
+        L.rpx_fill_const_f64(y.ptr, NE * D ** 3, 0.0, None)
+    L.rpx_diffusion3dpa_f64(Bm.ptr, G.ptr, Dd.ptr, X.ptr, Yf.ptr, NE, D, Q, 1, None, None)
+    L.rpx_diffusion3dpa_mixed_f64(Bm.ptr, G.ptr, Dd.ptr, X.ptr, Ym.ptr, NE, D, Q, 1, 7, scr.ptr, sb, None)
+    yf, ym = Yf.to_array(), Ym.to_array()
+    err = float(np.max(np.abs(ym - yf)) / np.max(np.abs(yf)))
+    _row(rows, "Apps_DIFFUSION3DPA(mixed-int8x7)", 4, ms_mix, nbytes, flops, err_vs_fp64_normwise=err,
+         speedup_vs_fp64=round(ms_f64 / ms_mix, 3), fp64_ms=round(ms_f64, 5))
 
 
 def mass_mixed(rows, reps):
