@@ -304,7 +304,10 @@ int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha,
  * it grows with the dynamic range within row i and column j.
  * nmoduli in [1, 16]; 0 picks the smallest count with b >= 42 (13 moduli,
  * b = 44 at nk = 4096).  nk is limited to 131071 (RPX_ERR_UNSUPPORTED
- * above), and to sizes where b >= 1. */
+ * above), and to sizes where b >= 1.  A's conversion runs on a
+ * library-owned auxiliary stream (created with two events on the first call
+ * on a device) forked from and joined back to `stream` with events, so the
+ * call stays ordered on `stream` and can be captured in a HIP graph. */
 size_t rpx_gemm_mixed_scratch_bytes(int64_t ni, int64_t nj, int64_t nk,
                                     int nmoduli);
 int rpx_gemm_mixed_bits(int64_t nk, int nmoduli);

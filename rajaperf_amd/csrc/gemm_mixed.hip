@@ -37,6 +37,8 @@
 // and the residue bytes d_t ([t] then the GEMM's accumulator fragments).
 #include "rpx_common.hpp"
 
+#include <mutex>
+
 using namespace rpx;
 
 namespace {
@@ -246,15 +248,21 @@ __global__ void fill_int_kernel(int* p, int64_t n, int v) {
 
 // B -> Bs[t][j][k]: a workgroup owns 64 columns x 128 k; work-item (w, j)
 // reads k0 + 16w .. k0 + 16w + 15 of column j (each load is 64 consecutive
-// columns of one row) and writes those 16 residues as one 16-byte chunk per
-// modulus, so no transpose through LDS is needed, and the workgroup's eight
-// chunks of a row make one whole 128-byte line.
+// columns of one row) and, per modulus, packs those 16 residues into one
+// 16-byte LDS write at row j, chunk w (rows padded to 144 bytes: the 16
+// writes of a quarter-wavefront fall in distinct bank groups).  Eight
+// work-items then read one 128-byte row back and store it as a whole line
+// of Bs, so every store instruction writes eight whole lines.
+constexpr int kColPitch = 144;  // bytes per LDS row of the B transpose
 __global__ __launch_bounds__(512) void split_cols_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
                                                          int64_t nkp, int64_t njp, const int* __restrict__ fb,
                                                          signed char* __restrict__ Bs, const CrtParams P) {
+  __shared__ __attribute__((aligned(16))) signed char lds[2][64 * kColPitch];
   const int tid = threadIdx.x;
-  const int64_t j = int64_t(blockIdx.x) * 64 + (tid & 63);
-  const int64_t k0 = int64_t(blockIdx.y) * 128 + 16 * (tid >> 6);
+  const int64_t j0 = int64_t(blockIdx.x) * 64;
+  const int64_t j = j0 + (tid & 63);
+  const int64_t kb = int64_t(blockIdx.y) * 128;
+  const int64_t k0 = kb + 16 * (tid >> 6);
   const int f = (j < nj) ? fb[j] : 0;
   const int sh = P.b - (f == INT32_MIN ? 0 : f);
   double x[16];
@@ -267,6 +275,8 @@ __global__ __launch_bounds__(512) void split_cols_kernel(const double* __restric
   }
 #pragma unroll
   for (int u = 0; u < 16; ++u) x[u] = rint(ldexp(x[u], sh));
+  const int jj = tid & 63, w16 = 16 * (tid >> 6);  // this work-item's LDS row and byte offset in it
+  const int rrow = tid >> 3, rpart = 16 * (tid & 7);  // the row chunk it stores
   for (int t = 0; t < P.T; ++t) {
     v4i w;
 #pragma unroll
@@ -279,7 +289,11 @@ __global__ __launch_bounds__(512) void split_cols_kernel(const double* __restric
       }
       w[c] = packed;
     }
-    *reinterpret_cast<v4i*>(Bs + (int64_t(t) * njp + j) * nkp + k0) = w;
+    signed char* buf = lds[t & 1];
+    *reinterpret_cast<v4i*>(buf + jj * kColPitch + w16) = w;
+    __syncthreads();  // the row is complete; buffer t & 1 was last read two moduli ago, before the previous barrier
+    const v4i row = *reinterpret_cast<const v4i*>(buf + rrow * kColPitch + rpart);
+    *reinterpret_cast<v4i*>(Bs + (int64_t(t) * njp + j0 + rrow) * nkp + kb + rpart) = row;
   }
 }
 
@@ -457,6 +471,34 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(double* __restrict__ C
   }
 }
 
+// A's row split does not depend on B's column exponents or B's split, so it
+// runs on a library-owned auxiliary stream (non-blocking, default
+// priority), forked from and joined back to the caller's stream with events;
+// stream capture follows the fork and join.  One auxiliary stream and event
+// pair per device, created on first use; the mutex keeps concurrent callers
+// from recording the shared events at the same time.
+struct AuxStream {
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kMaxDevices = 64;
+std::mutex g_aux_mu;
+AuxStream g_aux[kMaxDevices];
+
+int aux_for_device(AuxStream** out) {
+  int dev = -1;
+  RPX_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return RPX_ERR_UNSUPPORTED;
+  AuxStream& a = g_aux[dev];
+  if (!a.aux) {
+    RPX_TRY(hipStreamCreateWithFlags(&a.aux, hipStreamNonBlocking));
+    RPX_TRY(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming));
+    RPX_TRY(hipEventCreateWithFlags(&a.join, hipEventDisableTiming));
+  }
+  *out = &a;
+  return RPX_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -495,7 +537,14 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
   int* fb = reinterpret_cast<int*>(base + l.fb_off);
   unsigned* R = reinterpret_cast<unsigned*>(base + l.r_off);
   hipStream_t s = as_stream(stream);
-  split_rows_kernel<<<dim3(unsigned(l.nip)), dim3(256), 0, s>>>(A, ni, nk, l.nkp, l.nip, As, ea, P);
+  std::lock_guard<std::mutex> lock(g_aux_mu);
+  AuxStream* ax = nullptr;
+  if (const int st = aux_for_device(&ax); st != RPX_SUCCESS) return st;
+  RPX_TRY(hipEventRecord(ax->fork, s));
+  RPX_TRY(hipStreamWaitEvent(ax->aux, ax->fork, 0));
+  split_rows_kernel<<<dim3(unsigned(l.nip)), dim3(256), 0, ax->aux>>>(A, ni, nk, l.nkp, l.nip, As, ea, P);
+  if (const int st = launch_status(); st != RPX_SUCCESS) return st;
+  RPX_TRY(hipEventRecord(ax->join, ax->aux));
   fill_int_kernel<<<dim3(unsigned(ceil_div(l.njp, 256))), dim3(256), 0, s>>>(fb, l.njp, INT32_MIN);
   const int64_t kchunk = 64;
   if (nk > 0)
@@ -503,6 +552,7 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
         B, nk, nj, kchunk, fb);
   split_cols_kernel<<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 128)), dim3(512), 0, s>>>(B, nk, nj, l.nkp, l.njp,
                                                                                            fb, Bs, P);
+  RPX_TRY(hipStreamWaitEvent(s, ax->join, 0));
   ozaki_gemm_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(R, As, Bs, l.nip, l.njp, l.nkp, int(tn), P);
   crt_combine_kernel<<<dim3(unsigned(ceil_div(l.nip * l.njp / 4, 256))), dim3(256), 0, s>>>(
       C, R, ea, fb, alpha, ni, nj, l.nip, l.njp, int(tn), P);
