@@ -220,25 +220,37 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   }
 }
 
-// Column maxima of B as exponents: atomicMax over K chunks, 8 loads in
-// flight per work-item.
+// Column maxima of B as exponents.  A workgroup takes 64 columns x
+// kColExpRows rows: wavefront w reads rows w, w + 4, ... (each load is 64
+// consecutive columns of one row, 16 loads in flight per work-item), the
+// four wavefronts' maxima meet in LDS, and one atomicMax per column and
+// workgroup (nk / 512 per column) publishes the exponent.
+constexpr int kColExpRows = 512;
 __global__ __launch_bounds__(256) void col_exponent_kernel(const double* __restrict__ B, int64_t nk, int64_t nj,
-                                                           int64_t kchunk, int* __restrict__ fb) {
-  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (j >= nj) return;
-  const int64_t kb = int64_t(blockIdx.y) * kchunk;
-  const int64_t ke = (kb + kchunk < nk) ? kb + kchunk : nk;
+                                                           int* __restrict__ fb) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = int64_t(blockIdx.x) * 64 + lane;
+  const int64_t k0 = int64_t(blockIdx.y) * kColExpRows;
+  const int64_t ke = (k0 + kColExpRows < nk) ? k0 + kColExpRows : nk;
   double m = 0.0;
-  int64_t k = kb;
-  for (; k + 8 <= ke; k += 8) {
-    double v[8];
+  if (j < nj) {
+    int64_t k = k0 + w;
+    for (; k + 4 * 15 < ke; k += 4 * 16) {
+      double v[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = B[(k + u) * nj + j];
+      for (int u = 0; u < 16; ++u) v[u] = B[(k + 4 * u) * nj + j];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) m = fmax(m, fabs(v[u]));
+      for (int u = 0; u < 16; ++u) m = fmax(m, fabs(v[u]));
+    }
+    for (; k < ke; k += 4) m = fmax(m, fabs(B[k * nj + j]));
   }
-  for (; k < ke; ++k) m = fmax(m, fabs(B[k * nj + j]));
-  if (m != 0.0) atomicMax(&fb[j], exponent_of(m));
+  red[w][lane] = m;
+  __syncthreads();
+  if (w == 0 && j < nj) {
+    m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+    if (m != 0.0) atomicMax(&fb[j], exponent_of(m));
+  }
 }
 
 __global__ void fill_int_kernel(int* p, int64_t n, int v) {
@@ -334,7 +346,8 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
   const int64_t m0 = tm * kTM, n0 = tn * kTN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * kWaveM, wn = (wave & 1) * kWaveN;
-  const int64_t ksteps = nkp / kTK;
+  const int ksteps = int(nkp / kTK);
+  const int total = P.T * ksteps;  // stages over all moduli
   // Loader mapping: work-item tid moves chunk lc of rows lr + kRowStep*u, u < kLoads.
   constexpr int kRowStep = kThreads / kChunks;
   const int lc = tid % kChunks, lr = tid / kChunks;
@@ -342,40 +355,46 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
   // This wavefront's fragments in the residue array: [tile][wave][i][j][lane].
   const int64_t frag0 = ((tm * tiles_n + tn) * (kThreads / kWave) + wave) * (kFm * kFn);
   const int64_t per_mod = nip * njp / 4;  // 32-bit words per modulus
+  const signed char* a0 = As + (m0 + lr) * nkp + 16 * lc;
+  const signed char* b0 = Bs + (n0 + lr) * nkp + 16 * lc;
+  const int64_t a_mod = nip * nkp, b_mod = njp * nkp;
 
+  v4i ga[kLoads], gb[kLoads];
+  auto gload = [&](int g) {  // stage g: modulus g / ksteps, K step g % ksteps
+    const int tt = g / ksteps, st = g - tt * ksteps;
+    const signed char* at = a0 + tt * a_mod + st * kTK;
+    const signed char* bt = b0 + tt * b_mod + st * kTK;
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+      ga[u] = *reinterpret_cast<const v4i*>(at + int64_t(kRowStep * u) * nkp);
+      gb[u] = *reinterpret_cast<const v4i*>(bt + int64_t(kRowStep * u) * nkp);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+      Al[buf][swz(lr + kRowStep * u, lc)] = ga[u];
+      Bl[buf][swz(lr + kRowStep * u, lc)] = gb[u];
+    }
+  };
+
+  // The stages of all moduli form one pipeline: stage g + 1 is in flight in
+  // registers while stage g computes, so the first stage of the next
+  // modulus is loaded and stored like any other, and the load of the one
+  // after it is issued before the epilogue.
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  if (total > 1) gload(1);
+  int g = 0;
   for (int t = 0; t < P.T; ++t) {
     v4i acc[kFm][kFn];
 #pragma unroll
     for (int i = 0; i < kFm; ++i)
 #pragma unroll
       for (int j = 0; j < kFn; ++j) acc[i][j] = v4i{0, 0, 0, 0};
-
-    const signed char* at = As + (int64_t(t) * nip + m0 + lr) * nkp + 16 * lc;
-    const signed char* bt = Bs + (int64_t(t) * njp + n0 + lr) * nkp + 16 * lc;
-    v4i ga[kLoads], gb[kLoads];
-    auto gload = [&](int64_t st) {
-      const int64_t k0 = st * kTK;
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u) {
-        ga[u] = *reinterpret_cast<const v4i*>(at + int64_t(kRowStep * u) * nkp + k0);
-        gb[u] = *reinterpret_cast<const v4i*>(bt + int64_t(kRowStep * u) * nkp + k0);
-      }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u) {
-        Al[buf][swz(lr + kRowStep * u, lc)] = ga[u];
-        Bl[buf][swz(lr + kRowStep * u, lc)] = gb[u];
-      }
-    };
-
-    gload(0);
-    __syncthreads();  // the previous modulus's readers are done with buffer 0
-    lstore(0);
-    __syncthreads();
-    for (int64_t st = 0; st < ksteps; ++st) {
-      const int buf = int(st & 1);
-      if (st + 1 < ksteps) gload(st + 1);
+    for (int st = 0; st < ksteps; ++st, ++g) {
+      const int buf = g & 1;
       // Raised wave priority over the MFMA block: with two wavefronts per
       // SIMD the arbiter then keeps feeding the matrix pipe from one while
       // the other issues its loads and LDS stores.
@@ -395,11 +414,13 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
             acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
       }
       __builtin_amdgcn_s_setprio(0);
-      if (st + 1 < ksteps) {
-        lstore(buf ^ 1);
+      if (g + 1 < total) {
+        lstore(buf ^ 1);  // buffer buf ^ 1 was last read before the previous barrier
         __syncthreads();
       }
+      if (st + 1 < ksteps && g + 2 < total) gload(g + 2);
     }
+    if (g + 1 < total) gload(g + 1);  // g is the next modulus's first stage, already in LDS
     // d_t = the sum's symmetric residue modulo m_t (|sum| < 2^31, see
     // sym_mod); the four rows of a lane's accumulator go to one 32-bit word.
     const double m = P.m[t], inv = P.inv[t];
@@ -546,10 +567,9 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
   if (const int st = launch_status(); st != RPX_SUCCESS) return st;
   RPX_TRY(hipEventRecord(ax->join, ax->aux));
   fill_int_kernel<<<dim3(unsigned(ceil_div(l.njp, 256))), dim3(256), 0, s>>>(fb, l.njp, INT32_MIN);
-  const int64_t kchunk = 64;
   if (nk > 0)
-    col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 256)), unsigned(ceil_div(nk, kchunk))), dim3(256), 0, s>>>(
-        B, nk, nj, kchunk, fb);
+    col_exponent_kernel<<<dim3(unsigned(ceil_div(nj, 64)), unsigned(ceil_div(nk, kColExpRows))), dim3(256), 0, s>>>(
+        B, nk, nj, fb);
   split_cols_kernel<<<dim3(unsigned(l.njp / 64), unsigned(l.nkp / 128)), dim3(512), 0, s>>>(B, nk, nj, l.nkp, l.njp,
                                                                                            fb, Bs, P);
   RPX_TRY(hipStreamWaitEvent(s, ax->join, 0));
