@@ -58,11 +58,15 @@ int upload_async(const HostChunk* chunks, int n, hipStream_t s) {
     slot.device = dev;
   }
   if (slot.cap < total) {
+    // At least 64 KiB and at least twice the previous size, so that a run of
+    // calls with growing tables pins new memory only a few times.
+    size_t want = slot.cap * 2 > total ? slot.cap * 2 : total;
+    if (want < (size_t(64) << 10)) want = size_t(64) << 10;
     if (slot.host) RPX_TRY(hipHostFree(slot.host));
     slot.host = nullptr;
     slot.cap = 0;
-    RPX_TRY(hipHostMalloc(&slot.host, total, hipHostMallocDefault));
-    slot.cap = total;
+    RPX_TRY(hipHostMalloc(&slot.host, want, hipHostMallocDefault));
+    slot.cap = want;
   }
   char* p = static_cast<char*>(slot.host);
   size_t o = 0;

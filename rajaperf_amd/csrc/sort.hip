@@ -58,6 +58,13 @@ struct SegInfo {
   int pad;
 };
 
+// One per non-empty segment of a ragged layout, built on the host (O(nseg))
+// and expanded on the device into the per-tile TileInfo table.
+struct SegDesc {
+  long long seg_x, seg_alt, count_base;
+  int first_tile, ntiles, seg_len, pad;
+};
+
 // Uniform segmentation (equal lengths), or an explicit table in scratch.
 struct Plan {
   const TileInfo* tiles;  // null => uniform
@@ -322,6 +329,34 @@ __global__ __launch_bounds__(kBS) void scatter_kernel(const u64* __restrict__ sr
   }
 }
 
+// The ragged layout's tables from the per-segment descriptors: work-item t
+// writes TileInfo t (its segment found by binary search over the
+// descriptors' first tiles) and, for t < nsd, SegInfo t.
+__global__ __launch_bounds__(256) void tile_table_kernel(const SegDesc* __restrict__ d, int nsd,
+                                                         TileInfo* __restrict__ tiles, SegInfo* __restrict__ segs,
+                                                         int total_tiles) {
+  const int t = int(blockIdx.x) * 256 + int(threadIdx.x);
+  if (t < nsd) segs[t] = SegInfo{d[t].count_base, d[t].ntiles, 0};
+  if (t >= total_tiles) return;
+  int lo = 0, hi = nsd - 1;  // the last descriptor whose first tile is <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].first_tile <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const SegDesc sd = d[lo];
+  TileInfo ti;
+  ti.seg_x = sd.seg_x;
+  ti.seg_alt = sd.seg_alt;
+  ti.count_base = sd.count_base;
+  ti.local = t - sd.first_tile;
+  ti.ntiles = sd.ntiles;
+  ti.seg_len = sd.seg_len;
+  const int rem = sd.seg_len - ti.local * kTile;
+  ti.len = rem < kTile ? rem : kTile;
+  tiles[t] = ti;
+}
+
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 struct Sizes {
@@ -350,6 +385,7 @@ size_t scratch_for(const Sizes& s, int64_t nseg) {
   if (!s.uniform) {
     b += align256(size_t(s.total_tiles) * sizeof(TileInfo));
     b += align256(size_t(nseg) * sizeof(SegInfo));
+    b += align256(size_t(nseg) * sizeof(SegDesc));
   }
   return b;
 }
@@ -389,40 +425,35 @@ int rpx_sort_segmented_f64(double* x, const int64_t* offsets_host, int64_t nseg,
   int64_t nseg_launch = nseg;
   if (!s.uniform) {
     // Ragged layout: explicit tile and segment tables.  Segments keep their
-    // positions in x; in the scratch copy they are packed back to back.
-    std::vector<TileInfo> tiles;
-    std::vector<SegInfo> segs;
-    tiles.reserve(size_t(s.total_tiles));
+    // positions in x; in the scratch copy they are packed back to back.  The
+    // host describes each non-empty segment; a kernel expands the
+    // descriptors into one TileInfo per tile.
+    std::vector<SegDesc> desc;
+    desc.reserve(size_t(nseg));
     long long cbase = 0, packed = 0;
+    int first = 0;
     for (int64_t i = 0; i < nseg; ++i) {
       const long long len = offsets_host[i + 1] - offsets_host[i];
       const int nt = int((len + kTile - 1) / kTile);
       if (nt == 0) continue;
-      for (int t = 0; t < nt; ++t) {
-        TileInfo ti;
-        ti.seg_x = offsets_host[i];
-        ti.seg_alt = packed;
-        ti.count_base = cbase;
-        ti.local = t;
-        ti.ntiles = nt;
-        const long long rem = len - (long long)t * kTile;
-        ti.len = int(rem < kTile ? rem : kTile);
-        ti.seg_len = int(len);
-        tiles.push_back(ti);
-      }
-      segs.push_back(SegInfo{cbase, nt, 0});
+      desc.push_back(SegDesc{offsets_host[i], packed, cbase, first, nt, int(len), 0});
       cbase += (long long)nt * kRadix;
       packed += len;
+      first += nt;
     }
     TileInfo* dt = reinterpret_cast<TileInfo*>(base + off);
     off += align256(size_t(s.total_tiles) * sizeof(TileInfo));
     SegInfo* ds = reinterpret_cast<SegInfo*>(base + off);
-    const HostChunk tables[2] = {{dt, tiles.data(), tiles.size() * sizeof(TileInfo)},
-                                 {ds, segs.data(), segs.size() * sizeof(SegInfo)}};
-    if (const int rc = upload_async(tables, 2, st); rc != RPX_SUCCESS) return rc;
+    off += align256(size_t(nseg) * sizeof(SegInfo));
+    SegDesc* dd = reinterpret_cast<SegDesc*>(base + off);
+    const HostChunk table{dd, desc.data(), desc.size() * sizeof(SegDesc)};
+    if (const int rc = upload_async(&table, 1, st); rc != RPX_SUCCESS) return rc;
+    tile_table_kernel<<<dim3(unsigned(ceil_div(s.total_tiles, 256))), dim3(256), 0, st>>>(
+        dd, int(desc.size()), dt, ds, int(s.total_tiles));
+    if (const int rc = launch_status(); rc != RPX_SUCCESS) return rc;
     plan.tiles = dt;
     plan.segs = ds;
-    nseg_launch = int64_t(segs.size());
+    nseg_launch = int64_t(desc.size());
   }
 
   u64* xs = reinterpret_cast<u64*>(x);
