@@ -71,6 +71,10 @@ typedef struct rpx_tuning {
 #define RPX_TUNE_MASS_SINGLE_LAUNCH 0x8 /* rpx_mass3dpa_batched_run: one persistent
                                            launch for all orders instead of one per
                                            order present */
+#define RPX_TUNE_GEMM_BETA         0x10 /* rpx_gemm_f64, rpx_gemm_mixed_tuned_f64: the
+                                           textbook C = alpha*A*B + beta*C (each C
+                                           read once before it is written) instead of
+                                           the suite's C = alpha*A*B */
 
 /* ---- library / runtime support ---------------------------------------- *
  * Counterparts of the suite's HIP data utilities
@@ -283,7 +287,10 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps,
 /* GEMM with the suite's semantics (src/polybench/POLYBENCH_GEMM.hpp:12-50):
  *   C[i][j] = sum_k (alpha*A[i][k])*B[k][j]
  * `beta` is accepted for interface fidelity: the suite scales C by beta and
- * then overwrites it, so beta never reaches the result.  A is ni x nk, B is
+ * then overwrites it, so beta never reaches the result.  With
+ * RPX_TUNE_GEMM_BETA in tune->flags the result is instead the textbook
+ *   C[i][j] = sum_k (alpha*A[i][k])*B[k][j] + beta*C[i][j]
+ * (the sum as above, then beta*C added in FP64).  A is ni x nk, B is
  * nk x nj, C is ni x nj, all row-major.  FP64 on the MFMA f64 units;
  * RPX_TUNE_NAIVE selects the one-thread-per-element comparator. */
 int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha,
@@ -315,6 +322,14 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B,
                        double alpha, double beta, int64_t ni, int64_t nj,
                        int64_t nk, int nmoduli, void* scratch,
                        size_t scratch_bytes, void* stream);
+/* rpx_gemm_mixed_f64 with a tuning (src/polybench/POLYBENCH_GEMM.hpp:12-50
+ * as above): RPX_TUNE_GEMM_BETA selects the textbook C = alpha*A*B + beta*C,
+ * beta*C added in FP64 to the converted product; tune may be NULL. */
+int rpx_gemm_mixed_tuned_f64(double* C, const double* A, const double* B,
+                             double alpha, double beta, int64_t ni,
+                             int64_t nj, int64_t nk, int nmoduli,
+                             const rpx_tuning* tune, void* scratch,
+                             size_t scratch_bytes, void* stream);
 
 /* ---- apps (partial-assembly finite elements) -------------------------- */
 /* Mass operator action Y_e += Bt'^3 (D_e .* (B^3 X_e)) for NE hex elements

@@ -32,6 +32,7 @@ RPX_TUNE_CACHED = 0x1
 RPX_TUNE_NAIVE = 0x2
 RPX_TUNE_JACOBI_SKIP_B = 0x4
 RPX_TUNE_MASS_SINGLE_LAUNCH = 0x8
+RPX_TUNE_GEMM_BETA = 0x10
 
 
 class ExtensionMissing(RuntimeError):
@@ -116,6 +117,7 @@ _SIGS = {
     "rpx_gemm_mixed_scratch_bytes": (_sz, [_i64, _i64, _i64, _i32]),
     "rpx_gemm_mixed_bits": (_i32, [_i64, _i32]),
     "rpx_gemm_mixed_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _i32, _vp, _sz, _vp]),
+    "rpx_gemm_mixed_tuned_f64": (_i32, [_vp, _vp, _vp, _d, _d, _i64, _i64, _i64, _i32, _tp, _vp, _sz, _vp]),
     "rpx_mass3dpa_f64": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _tp, _vp]),
     "rpx_mass3dpa_batched_scratch_bytes": (_sz, [C.POINTER(MassBatch), _i32]),
     "rpx_mass3dpa_batched_plan": (_i32, [C.POINTER(MassBatch), _i32, _vp, _sz, _vp]),

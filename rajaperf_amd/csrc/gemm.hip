@@ -1,4 +1,7 @@
-// GEMM (polybench): C[i][j] = sum_k (alpha*A[i][k]) * B[k][j].
+// GEMM (polybench): C[i][j] = sum_k (alpha*A[i][k]) * B[k][j].  With
+// RPX_TUNE_GEMM_BETA the textbook form C = alpha*A*B + beta*C instead (the
+// old C read once; the MFMA kernel starts its sums at beta*C, the
+// comparator adds beta*C to its sum).
 //
 // Reference: src/polybench/POLYBENCH_GEMM.hpp:12-50 (C *= beta, then C is
 // overwritten by the dot, so beta never reaches the result),
@@ -25,14 +28,14 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 // ---- plain FP64 comparator ---------------------------------------------------
 __global__ __launch_bounds__(256) void gemm_naive_kernel(double* __restrict__ C, const double* __restrict__ A,
-                                                         const double* __restrict__ B, double alpha, int64_t ni,
-                                                         int64_t nj, int64_t nk) {
+                                                         const double* __restrict__ B, double alpha, double beta,
+                                                         int use_beta, int64_t ni, int64_t nj, int64_t nk) {
   const int64_t j = int64_t(blockIdx.x) * 32 + (threadIdx.x & 31);
   const int64_t i = int64_t(blockIdx.y) * 8 + (threadIdx.x >> 5);
   if (i >= ni || j >= nj) return;
   double dot = 0.0;
   for (int64_t k = 0; k < nk; ++k) dot = fma(alpha * A[k + i * nk], B[j + k * nj], dot);
-  C[j + i * nj] = dot;
+  C[j + i * nj] = use_beta ? dot + beta * C[j + i * nj] : dot;
 }
 
 // ---- FP64 MFMA ---------------------------------------------------------------
@@ -51,11 +54,12 @@ constexpr int kTM = 128, kTN = 128, kTK = 16;
 constexpr int kThreads = 256;
 constexpr int kLdsRow = kTK + 2;  // doubles per LDS row of either operand
 
-__global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restrict__ C,
-                                                                 const double* __restrict__ A,
-                                                                 const double* __restrict__ B, double alpha,
-                                                                 int64_t ni, int64_t nj, int64_t nk,
-                                                                 int tiles_n, int vec) {
+// BETA: the textbook C = beta*C + the products (a separate kernel; the
+// suite's form reads no C).
+template <bool BETA>
+__device__ __forceinline__ void gemm_mfma_f64_tile(double* __restrict__ C, const double* __restrict__ A,
+                                                   const double* __restrict__ B, double alpha, double beta,
+                                                   int64_t ni, int64_t nj, int64_t nk, int tiles_n, int vec) {
   __shared__ __attribute__((aligned(16))) double As[2][kTM * kLdsRow];  // As[m][k]
   __shared__ __attribute__((aligned(16))) double Bs[2][kTN * kLdsRow];  // Bs[n][k]
 
@@ -119,11 +123,32 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
       *reinterpret_cast<v2d*>(&Bs[buf][(b_n + c) * kLdsRow + b_k]) = v2d{rb[c * 2], rb[c * 2 + 1]};
   };
 
+  // With BETA the accumulators start at beta*C (the old C read once, before
+  // the K loop), so the result is beta*C plus the products; otherwise at 0.
+  // The C reads go one fragment row at a time (the empty asm statement keeps
+  // the compiler from issuing all 64 together, which took 100 more VGPRs
+  // and a wavefront per SIMD).
   v4d acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; ++j) {
+      if constexpr (BETA) {
+        if (j == 0) asm volatile("" ::: "memory");
+      }
+      acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      if constexpr (BETA) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gi = m0 + wm + i * 16 + 4 * r + (lane >> 4);
+          const int64_t gj = n0 + wn + j * 16 + (lane & 15);
+          // Clamped into C: the accumulators of rows or columns outside C
+          // are never stored, so what they start from does not matter.
+          const int64_t ci = gi < ni ? gi : ni - 1, cj = gj < nj ? gj : nj - 1;
+          acc[i][j][r] = beta * C[ci * nj + cj];
+        }
+      }
+    }
 
   const int64_t nslabs = (nk + kTK - 1) / kTK;
   load_slab(0);
@@ -165,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
   // v_mfma_f64_16x16x4f64 accumulator layout (measured on gfx950): element r
   // of lane l is row 4*r + l/16, column l%16.
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -174,28 +199,52 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restr
         const int64_t gj = n0 + wn + j * 16 + (lane & 15);
         if (gi < ni && gj < nj) C[gi * nj + gj] = acc[i][j][r];
       }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void gemm_mfma_f64_kernel(double* __restrict__ C,
+                                                                 const double* __restrict__ A,
+                                                                 const double* __restrict__ B, double alpha,
+                                                                 int64_t ni, int64_t nj, int64_t nk, int tiles_n,
+                                                                 int vec) {
+  gemm_mfma_f64_tile<false>(C, A, B, alpha, 0.0, ni, nj, nk, tiles_n, vec);
+}
+
+// The textbook form.  Left to itself the compiler gives this instance 256
+// VGPRs and 148 AGPRs (one wavefront per SIMD, 1.6x the time); capped at
+// two wavefronts per SIMD it spills 20 bytes per work-item.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_mfma_f64_beta_kernel(
+    double* __restrict__ C, const double* __restrict__ A, const double* __restrict__ B, double alpha, double beta,
+    int64_t ni, int64_t nj, int64_t nk, int tiles_n, int vec) {
+  gemm_mfma_f64_tile<true>(C, A, B, alpha, beta, ni, nj, nk, tiles_n, vec);
 }
 
 }  // namespace
 
 extern "C" {
 
-int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha, double /*beta*/, int64_t ni,
+int rpx_gemm_f64(double* C, const double* A, const double* B, double alpha, double beta, int64_t ni,
                  int64_t nj, int64_t nk, const rpx_tuning* tune, void* stream) {
   if (ni < 0 || nj < 0 || nk < 0) return RPX_ERR_INVALID_ARGUMENT;
   if (ni == 0 || nj == 0) return RPX_SUCCESS;
   if (!C || (nk > 0 && (!A || !B))) return RPX_ERR_INVALID_ARGUMENT;
   hipStream_t s = as_stream(stream);
+  const int use_beta = (tune_flags(tune) & RPX_TUNE_GEMM_BETA) != 0;
   if (tune_flags(tune) & RPX_TUNE_NAIVE) {
     const int64_t gx = ceil_div(nj, 32), gy = ceil_div(ni, 8);
     if (gx > INT32_MAX || gy > 65535) return RPX_ERR_UNSUPPORTED;
-    gemm_naive_kernel<<<dim3(unsigned(gx), unsigned(gy)), dim3(256), 0, s>>>(C, A, B, alpha, ni, nj, nk);
+    gemm_naive_kernel<<<dim3(unsigned(gx), unsigned(gy)), dim3(256), 0, s>>>(C, A, B, alpha, beta, use_beta, ni,
+                                                                              nj, nk);
     return launch_status();
   }
   const int64_t tm = ceil_div(ni, kTM), tn = ceil_div(nj, kTN);
   if (tm * tn > INT32_MAX || tn > INT32_MAX) return RPX_ERR_UNSUPPORTED;
   const int vec = (nk % 2 == 0) && (nj % 2 == 0) && aligned16(A) && aligned16(B);
-  gemm_mfma_f64_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, A, B, alpha, ni, nj, nk, int(tn),
+  if (use_beta)
+    gemm_mfma_f64_beta_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, A, B, alpha, beta, ni, nj, nk,
+                                                                                int(tn), vec);
+  else
+    gemm_mfma_f64_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(C, A, B, alpha, ni, nj, nk, int(tn),
                                                                            vec);
   return launch_status();
 }

@@ -443,8 +443,9 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
 // modulo 1 after each add).
 __global__ __launch_bounds__(256) void crt_combine_kernel(double* __restrict__ C, const unsigned* __restrict__ R,
                                                           const int* __restrict__ ea, const int* __restrict__ fb,
-                                                          double alpha, int64_t ni, int64_t nj, int64_t nip,
-                                                          int64_t njp, int tiles_n, const CrtParams P) {
+                                                          double alpha, double beta, int use_beta, int64_t ni,
+                                                          int64_t nj, int64_t nip, int64_t njp, int tiles_n,
+                                                          const CrtParams P) {
   const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
   const int64_t per_mod = nip * njp / 4;
   if (g >= per_mod) return;
@@ -488,7 +489,9 @@ __global__ __launch_bounds__(256) void crt_combine_kernel(double* __restrict__ C
     const double p = fx * P.M_hi;
     const double e = fma(fx, P.M_hi, -p);
     const double x = p + (e + fma(fx, P.M_lo, fl[r] * P.M_hi));
-    C[row * nj + col] = alpha * ldexp(x, ea[row] + sc);
+    const double v = alpha * ldexp(x, ea[row] + sc);
+    double* c = C + row * nj + col;
+    *c = use_beta ? v + beta * *c : v;
   }
 }
 
@@ -538,8 +541,15 @@ int rpx_gemm_mixed_bits(int64_t nk, int nmoduli) {
   return P.b;
 }
 
-int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha, double /*beta*/, int64_t ni,
+int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha, double beta, int64_t ni,
                        int64_t nj, int64_t nk, int nmoduli, void* scratch, size_t scratch_bytes, void* stream) {
+  return rpx_gemm_mixed_tuned_f64(C, A, B, alpha, beta, ni, nj, nk, nmoduli, nullptr, scratch, scratch_bytes,
+                                  stream);
+}
+
+int rpx_gemm_mixed_tuned_f64(double* C, const double* A, const double* B, double alpha, double beta, int64_t ni,
+                             int64_t nj, int64_t nk, int nmoduli, const rpx_tuning* tune, void* scratch,
+                             size_t scratch_bytes, void* stream) {
   if (ni < 0 || nj < 0 || nk < 0 || nmoduli < 0 || nmoduli > kMaxMods) return RPX_ERR_INVALID_ARGUMENT;
   if (ni == 0 || nj == 0) return RPX_SUCCESS;
   if (!C || !scratch || (nk > 0 && (!A || !B))) return RPX_ERR_INVALID_ARGUMENT;
@@ -574,8 +584,9 @@ int rpx_gemm_mixed_f64(double* C, const double* A, const double* B, double alpha
                                                                                            fb, Bs, P);
   RPX_TRY(hipStreamWaitEvent(s, ax->join, 0));
   ozaki_gemm_kernel<<<dim3(unsigned(tm * tn)), dim3(kThreads), 0, s>>>(R, As, Bs, l.nip, l.njp, l.nkp, int(tn), P);
+  const int use_beta = (tune_flags(tune) & RPX_TUNE_GEMM_BETA) != 0;
   crt_combine_kernel<<<dim3(unsigned(ceil_div(l.nip * l.njp / 4, 256))), dim3(256), 0, s>>>(
-      C, R, ea, fb, alpha, ni, nj, l.nip, l.njp, int(tn), P);
+      C, R, ea, fb, alpha, beta, use_beta, ni, nj, l.nip, l.njp, int(tn), P);
   return launch_status();
 }
 

@@ -217,6 +217,33 @@ def config4(rows, reps, sample_rows=8):
              int8_tops=round(nmod * flops / (ms_mix / 1e3) / 1e12, 1),
              speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
         del scr
+    # The textbook form C = alpha*A*B + beta*C0 (RPX_TUNE_GEMM_BETA), a
+    # separately labelled output: errors from one call on C0, against
+    # 80-bit host arithmetic relative to |alpha|(|A||B|) + |beta C0|; times
+    # over repeated calls (C then carries the previous result).
+    beta = 1.2
+    C0 = B(8 * n * n)
+    L.rpx_fill_uniform_f64(C0.ptr, n * n, SEED, 4, -1.0, 1.0, None)
+    c0 = C0.to_array().reshape(n, n)[rows_idx]
+    ref_tb = ref + np.longdouble(beta) * c0.astype(np.longdouble)
+    scale_tb = scale + abs(beta) * np.abs(c0)
+    tb = abi.tuning(flags=abi.RPX_TUNE_GEMM_BETA)
+    L.rpx_memcpy_d2d(Cf.ptr, C0.ptr, 8 * n * n, None)
+    L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, tb, None)
+    err_tb = float(np.max(np.abs(Cf.to_array().reshape(n, n)[rows_idx] - ref_tb) / scale_tb))
+    ms_tb = _time(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, tb, None), reps, warmup=10)
+    _row(rows, "Polybench_GEMM(fp64-mfma, textbook alpha*AB+beta*C)", 4, ms_tb, None, flops, "mfma", FP64_TFLOPS,
+         err_vs_exact=err_tb)
+    sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 0)
+    scr = B(sb)
+    L.rpx_memcpy_d2d(Cm.ptr, C0.ptr, 8 * n * n, None)
+    L.rpx_gemm_mixed_tuned_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, 0, tb, scr.ptr, sb, None)
+    err_tbm = float(np.max(np.abs(Cm.to_array().reshape(n, n)[rows_idx] - ref_tb) / scale_tb))
+    ms_tbm = _time(lambda: L.rpx_gemm_mixed_tuned_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, 0, tb, scr.ptr,
+                                                      sb, None), reps, warmup=10)
+    _row(rows, "Polybench_GEMM(mixed-int8-13mod, textbook alpha*AB+beta*C)", 4, ms_tbm, None, flops, "mfma",
+         FP64_TFLOPS, err_vs_exact=err_tbm, speedup_vs_fp64_mfma=round(ms_tb / ms_tbm, 3))
+    del scr, C0
     del dA, dB, Cf, Cm
     mass_mixed(rows, reps)
     diffusion_mixed(rows, reps)

@@ -44,6 +44,38 @@ def test_gemm_identity_exact(rpx, rng):
     assert np.array_equal(dC.to_array(), A)
 
 
+@pytest.mark.parametrize("naive", [False, True])
+@pytest.mark.parametrize("ni,nj,nk", [(1, 1, 1), (37, 53, 71), (128, 128, 64), (130, 257, 300)])
+def test_gemm_f64_textbook_beta(rpx, orc, rng, ni, nj, nk, naive):
+    """RPX_TUNE_GEMM_BETA: C = alpha*A*B + beta*C.  Checked against the
+    oracle's product (the suite's form) plus beta*C0 in 80-bit arithmetic; a
+    zero A gives beta*C0 exactly, and beta = 0 gives the suite's result bit
+    for bit."""
+    A = rng.uniform(-1, 1, ni * nk)
+    B = rng.uniform(-1, 1, nk * nj)
+    C0 = rng.uniform(-1, 1, ni * nj)
+    alpha, beta = 1.5, 1.2
+    flags = rpx.RPX_TUNE_GEMM_BETA | (rpx.RPX_TUNE_NAIVE if naive else 0)
+    L = rpx.lib()
+    dA, dB = dev(rpx, A), dev(rpx, B)
+    print(f"textbook GEMM {ni}x{nj}x{nk} naive={naive}", flush=True)
+
+    def run(a_buf, beta_, fl):
+        dC = dev(rpx, C0)
+        L.rpx_gemm_f64(dC.ptr, a_buf.ptr, dB.ptr, alpha, beta_, ni, nj, nk, rpx.tuning(flags=fl), None)
+        return dC.to_array().reshape(ni, nj)
+
+    got = run(dA, beta, flags)
+    prod = orc.gemm(A, B, alpha, beta, ni, nj, nk).reshape(ni, nj).astype(np.longdouble)
+    c0 = C0.reshape(ni, nj)
+    ref = prod + np.longdouble(beta) * c0.astype(np.longdouble)
+    scale = abs(alpha) * (np.abs(A.reshape(ni, nk)) @ np.abs(B.reshape(nk, nj))) + abs(beta) * np.abs(c0)
+    assert np.all(np.abs(got.astype(np.longdouble) - ref) <= 2 * (nk + 2) * EPS * scale + 1e-300)
+    assert np.array_equal(run(dev(rpx, np.zeros(ni * nk)), beta, flags), beta * c0)
+    suite = run(dA, beta, flags & ~rpx.RPX_TUNE_GEMM_BETA)
+    assert np.array_equal(run(dA, 0.0, flags), suite)
+
+
 MASS_ORDERS = [(d, q) for d in range(1, 9) for q in (d, d + 1, d + 2) if q <= 9]
 
 

@@ -65,6 +65,39 @@ def test_mixed_gemm_meets_1e12(rpx, rng, ni, nj, nk):
     assert np.all(np.abs(C - exact(A, B, 1.5, ni, nj, nk)) <= bound)
 
 
+@pytest.mark.parametrize("ni,nj,nk", [(64, 96, 200), (300, 257, 513)])
+def test_mixed_gemm_textbook_beta(rpx, rng, ni, nj, nk):
+    """rpx_gemm_mixed_tuned_f64 with RPX_TUNE_GEMM_BETA: C = alpha*A*B +
+    beta*C0 within the product's stated bound plus the FP64 rounding of the
+    sum; without the flag (or through rpx_gemm_mixed_f64) C0 is ignored and
+    both entry points agree bit for bit."""
+    A = rng.uniform(-1, 1, ni * nk)
+    B = rng.uniform(-1, 1, nk * nj)
+    C0 = rng.uniform(-1, 1, ni * nj)
+    alpha, beta = 1.5, 1.2
+    L = rpx.lib()
+    sb = L.rpx_gemm_mixed_scratch_bytes(ni, nj, nk, 0)
+    scr = rpx.DeviceBuffer(sb)
+    dA, dB = dev(rpx, A), dev(rpx, B)
+    print(f"mixed textbook GEMM {ni}x{nj}x{nk}", flush=True)
+
+    def run(tune):
+        dC = dev(rpx, C0)
+        L.rpx_gemm_mixed_tuned_f64(dC.ptr, dA.ptr, dB.ptr, alpha, beta, ni, nj, nk, 0, tune, scr.ptr, sb, None)
+        return dC.to_array().reshape(ni, nj)
+
+    got = run(rpx.tuning(flags=rpx.RPX_TUNE_GEMM_BETA))
+    c0 = C0.reshape(ni, nj).astype(np.longdouble)
+    ref = exact(A, B, alpha, ni, nj, nk) + np.longdouble(beta) * c0
+    bound, _ = stated_bound(rpx, A, B, alpha, ni, nj, nk)
+    bound = bound + 4 * EPS * (abs(beta) * np.abs(C0.reshape(ni, nj)) + np.abs(ref.astype(np.float64)))
+    assert np.all(np.abs(got.astype(np.longdouble) - ref) <= bound)
+    plain = run(None)
+    assert np.array_equal(plain, run_mixed(rpx, A, B, alpha, ni, nj, nk))
+    assert np.all(np.abs(plain.astype(np.longdouble) - exact(A, B, alpha, ni, nj, nk))
+                  <= stated_bound(rpx, A, B, alpha, ni, nj, nk)[0])
+
+
 def test_mixed_gemm_wide_dynamic_range(rpx, rng):
     ni, nj, nk = 96, 80, 300
     A = rng.uniform(-1, 1, ni * nk) * np.exp2(rng.integers(-30, 30, ni * nk))
