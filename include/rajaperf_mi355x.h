@@ -217,6 +217,13 @@ int rpx_scan_scratch_status(const void* scratch);
 int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n,
                            void* scratch, size_t scratch_bytes,
                            const rpx_tuning* tune, void* stream);
+/* Inclusive form, y[i] = x[0] + ... + x[i] (each element added to its
+ * exclusive value), an extra output beside the suite's exclusive SCAN
+ * (src/algorithm/SCAN.hpp:27-32 defines only the exclusive one).  Same
+ * scratch, tunings and conventions as rpx_scan_exclusive_f64. */
+int rpx_scan_inclusive_f64(const double* x, double* y, int64_t n,
+                           void* scratch, size_t scratch_bytes,
+                           const rpx_tuning* tune, void* stream);
 /* Segmented form: segment s covers [offsets[s], offsets[s+1]) and is scanned
  * independently; `offsets_host` is a host array of nseg+1 ascending values.
  * All segments run in one launch (tiles of 16384 doubles, the look-back of

@@ -91,6 +91,8 @@ typedef struct orc_params {
   int32_t d1d, q1d;     /* FEM orders; <= 0 = suite defaults                */
   int32_t nthreads;     /* Base_OpenMP threads; <= 0 = OpenMP default       */
   int32_t pad_;
+  double alpha, beta;   /* GEMM scalars when has_alpha / has_beta are set   */
+  int32_t has_alpha, has_beta;
 } orc_params;
 
 typedef struct orc_result {

@@ -22,7 +22,8 @@ class Params(C.Structure):
     _fields_ = [("input_mode", C.c_int32), ("npasses", C.c_int32), ("seed", C.c_uint64),
                 ("size", C.c_int64), ("sizefact", C.c_double), ("repfact", C.c_double),
                 ("checkrun_reps", C.c_int64), ("tsteps", C.c_int64), ("nk", C.c_int64),
-                ("d1d", C.c_int32), ("q1d", C.c_int32), ("nthreads", C.c_int32), ("pad_", C.c_int32)]
+                ("d1d", C.c_int32), ("q1d", C.c_int32), ("nthreads", C.c_int32), ("pad_", C.c_int32),
+                ("alpha", C.c_double), ("beta", C.c_double), ("has_alpha", C.c_int32), ("has_beta", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -260,8 +261,10 @@ def diffusion3dpa(Basis, dBasis, D, X, Y, NE, d1d, q1d, symmetric=True, variant=
 
 def run(kernel: str, variant: str = "Base_Seq", *, input_mode: int = 0, seed: int = 42, size: int = 0,
         sizefact: float = 1.0, repfact: float = 1.0, checkrun: int = 0, npasses: int = 1,
-        tsteps: int = 0, nk: int = 0, d1d: int = 0, q1d: int = 0, nthreads: int = 0) -> dict:
-    p = Params(input_mode, npasses, seed, size, sizefact, repfact, checkrun, tsteps, nk, d1d, q1d, nthreads, 0)
+        tsteps: int = 0, nk: int = 0, d1d: int = 0, q1d: int = 0, nthreads: int = 0,
+        alpha: float | None = None, beta: float | None = None) -> dict:
+    p = Params(input_mode, npasses, seed, size, sizefact, repfact, checkrun, tsteps, nk, d1d, q1d, nthreads, 0,
+               0.0 if alpha is None else alpha, 0.0 if beta is None else beta, alpha is not None, beta is not None)
     r = Result()
     rc = lib().orc_run(kernel.encode(), variant.encode(), C.byref(p), C.byref(r))
     if rc != 0:

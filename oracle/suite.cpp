@@ -219,8 +219,8 @@ long double run_gemm(const Ctx& c, int vid, orc_result* r, Timer& t) {
   const int64_t nj = ni;
   const int64_t nk = c.p->nk > 0 ? c.p->nk : static_cast<int64_t>(double(1200) / 1000 * ni);
   const int64_t reps = c.reps(4);
-  const double alpha = c.seeded() ? 1.5 : 0.62;
-  const double beta = c.seeded() ? 1.2 : 1.002;
+  const double alpha = c.p->has_alpha ? c.p->alpha : (c.seeded() ? 1.5 : 0.62);  // or --alpha / --beta
+  const double beta = c.p->has_beta ? c.p->beta : (c.seeded() ? 1.2 : 1.002);
   r->problem_size = ni * nj;
   r->reps = reps;
   r->bytes_per_rep = 8 * ni * nk + 8 * nj * nk + 8 * ni * nj;

@@ -103,6 +103,7 @@ _SIGS = {
     "rpx_scan_scratch_init": (_i32, [_vp, _sz, _vp]),
     "rpx_scan_scratch_status": (_i32, [_vp]),
     "rpx_scan_exclusive_f64": (_i32, [_vp, _vp, _i64, _vp, _sz, _tp, _vp]),
+    "rpx_scan_inclusive_f64": (_i32, [_vp, _vp, _i64, _vp, _sz, _tp, _vp]),
     "rpx_scan_segmented_scratch_bytes": (_sz, [_i64p, _i64]),
     "rpx_scan_exclusive_segmented_f64": (_i32, [_vp, _vp, _i64p, _i64, _vp, _sz, _vp]),
     "rpx_sort_segmented_scratch_bytes": (_sz, [_i64p, _i64]),

@@ -1,4 +1,5 @@
-// SCAN: exclusive FP64 prefix sum in one pass with decoupled look-back.
+// SCAN: exclusive FP64 prefix sum in one pass with decoupled look-back (and
+// the inclusive form as an extra output).
 //
 // Reference: src/algorithm/SCAN.hpp:27-32 (y[i] = s; s += x[i]),
 // SCAN-Seq.cpp:31-45, SCAN-OMP.cpp:33-90, SCAN-Hip.cpp:39-326 and the
@@ -128,7 +129,7 @@ struct SegTables {
 // carry the epoch read from the header.  SEG = true: tiles come from the
 // segment tables, the look-back stops at the segment's first tile, and the
 // flags were cleared before the launch (epoch 0, no epoch advance).
-template <int BS, int ROWS, bool NT, bool SEG>  // 16-byte rows per lane: a wavefront scans 128*ROWS doubles
+template <int BS, int ROWS, bool NT, bool SEG, bool INCL = false>  // 16-byte rows per lane: 128*ROWS doubles a wave
 __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, int64_t n, int64_t ntiles,
                                                    ScanHeader* __restrict__ hdr,
                                                    unsigned long long* __restrict__ flags,
@@ -183,7 +184,8 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
   }
 
   // Exclusive scan inside the wavefront, row by row: lane pair sums, a
-  // 64-lane scan on DPP moves, and a running carry of the row totals.
+  // 64-lane scan on DPP moves, and a running carry of the row totals.  The
+  // inclusive form (INCL) adds each element to its exclusive value.
   double carry = 0.0;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
@@ -192,7 +194,11 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
     const double exc = wave_shift_up_dpp(inc);
     const double row_total = last_lane(inc);
     const double e0 = carry + exc;
-    v[r] = v2d{e0, e0 + v[r].x};
+    const double e1 = e0 + v[r].x;
+    if constexpr (INCL)
+      v[r] = v2d{e1, e1 + v[r].y};
+    else
+      v[r] = v2d{e0, e1};
     carry += row_total;
   }
   if (lane == 0) wave_tot[wave] = carry;
@@ -304,17 +310,18 @@ Layout layout_for(int64_t ntiles) {
   return l;
 }
 
-template <int ROWS, bool SEG, int BS = kBS>
+template <int ROWS, bool SEG, int BS = kBS, bool INCL = false>
 void launch_rows(bool nt, unsigned grid, const double* x, double* y, int64_t n, int64_t ntiles, ScanHeader* hdr,
                  unsigned long long* flags, double* aggs, double* incls, int vec, SegTables seg, hipStream_t s) {
   if (nt)
-    scan_kernel<BS, ROWS, true, SEG><<<dim3(grid), dim3(BS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec,
-                                                                      seg);
+    scan_kernel<BS, ROWS, true, SEG, INCL><<<dim3(grid), dim3(BS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls,
+                                                                            vec, seg);
   else
-    scan_kernel<BS, ROWS, false, SEG><<<dim3(grid), dim3(BS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls, vec,
-                                                                       seg);
+    scan_kernel<BS, ROWS, false, SEG, INCL><<<dim3(grid), dim3(BS), 0, s>>>(x, y, n, ntiles, hdr, flags, aggs, incls,
+                                                                             vec, seg);
 }
 
+template <bool INCL>
 int scan_one(const double* x, double* y, int64_t n, int ipt, int bs, bool nt, void* scratch, size_t scratch_bytes,
              hipStream_t s) {
   if (n == 0) return RPX_SUCCESS;
@@ -333,13 +340,15 @@ int scan_one(const double* x, double* y, int64_t n, int ipt, int bs, bool nt, vo
   const unsigned grid = unsigned(ntiles);
   const SegTables none{nullptr, nullptr, nullptr};
   if (bs == 512) {
-    if (ipt == 16) launch_rows<8, false, 512>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
-    else if (ipt == 32) launch_rows<16, false, 512>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
-    else launch_rows<32, false, 512>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    if (ipt == 16) launch_rows<8, false, 512, INCL>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else if (ipt == 32)
+      launch_rows<16, false, 512, INCL>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else launch_rows<32, false, 512, INCL>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
   } else {
-    if (ipt == 16) launch_rows<8, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
-    else if (ipt == 32) launch_rows<16, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
-    else launch_rows<32, false>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    if (ipt == 16) launch_rows<8, false, kBS, INCL>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else if (ipt == 32)
+      launch_rows<16, false, kBS, INCL>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
+    else launch_rows<32, false, kBS, INCL>(nt, grid, x, y, n, ntiles, hdr, flags, aggs, incls, vec, none, s);
   }
   return launch_status();
 }
@@ -383,7 +392,16 @@ int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n, void* scratch,
   const int ipt = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultIPT);
   const int bs = tune_or(tune, &rpx_tuning::block_size, kDefaultScanBS);
   const bool nt = (tune_flags(tune) & RPX_TUNE_CACHED) == 0;
-  return scan_one(x, y, n, ipt, bs, nt, scratch, scratch_bytes, as_stream(stream));
+  return scan_one<false>(x, y, n, ipt, bs, nt, scratch, scratch_bytes, as_stream(stream));
+}
+
+int rpx_scan_inclusive_f64(const double* x, double* y, int64_t n, void* scratch, size_t scratch_bytes,
+                           const rpx_tuning* tune, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !y || !scratch))) return RPX_ERR_INVALID_ARGUMENT;
+  const int ipt = tune_or(tune, &rpx_tuning::items_per_thread, kDefaultIPT);
+  const int bs = tune_or(tune, &rpx_tuning::block_size, kDefaultScanBS);
+  const bool nt = (tune_flags(tune) & RPX_TUNE_CACHED) == 0;
+  return scan_one<true>(x, y, n, ipt, bs, nt, scratch, scratch_bytes, as_stream(stream));
 }
 
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host, int64_t nseg) {

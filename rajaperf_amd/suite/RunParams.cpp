@@ -1,5 +1,6 @@
 #include "RunParams.hpp"
 
+#include <cmath>
 #include <cstdlib>
 #include <iostream>
 #include <sstream>
@@ -147,6 +148,12 @@ void RunParams::parse(int argc, const char** argv) {
       if (need(i + 1 < argc && parse_int(argv[i + 1], iv) && iv >= 0)) tsteps = iv, ++i;
     } else if (opt == "--nk") {
       if (need(i + 1 < argc && parse_int(argv[i + 1], iv) && iv >= 0)) nk = iv, ++i;
+    } else if (opt == "--alpha") {
+      if (need(i + 1 < argc && parse_double(argv[i + 1], dv) && std::isfinite(dv)))
+        gemm_alpha = dv, has_alpha = true, ++i;
+    } else if (opt == "--beta") {
+      if (need(i + 1 < argc && parse_double(argv[i + 1], dv) && std::isfinite(dv)))
+        gemm_beta = dv, has_beta = true, ++i;
     } else if (opt == "--d1d") {
       if (need(i + 1 < argc && parse_int(argv[i + 1], iv) && iv >= 1 && iv <= 8)) d1d = int(iv), ++i;
     } else if (opt == "--q1d") {
@@ -248,7 +255,7 @@ void RunParams::printHelpMessage(std::ostream& str) const {
       << "  --gpu_block_size <n>        workgroup size for the default tuning\n"
       << "  --items_per_thread <n>      items per work-item for the default tuning\n"
       << "  --input-mode reference|seeded, --seed <s>\n"
-      << "  --tsteps <n> (JACOBI_2D), --nk <n> (GEMM), --d1d/--q1d <n> (FEM)\n"
+      << "  --tsteps <n> (JACOBI_2D), --nk <n>, --alpha <x>, --beta <x> (GEMM), --d1d/--q1d <n> (FEM)\n"
       << "  -od/--outdir <dir>, -of/--outfile <prefix>, --json <file>\n"
       << "  --tuning-cache <file>  (decisions of the \"auto\" tunings)\n"
       << "  --progress <file.jsonl> [--resume]  one line per finished run; resume skips recorded runs\n"

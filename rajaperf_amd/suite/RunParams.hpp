@@ -7,7 +7,7 @@
 // --items_per_thread, -od/--outdir, -of/--outfile, --refvar, --dryrun,
 // --disable-warmup, --show-progress, --print-kernels, --print-variants,
 // -h/--help.  Added for this variant: --input-mode {reference,seeded},
-// --seed, --tsteps (JACOBI_2D), --nk (GEMM), --d1d/--q1d (FEM orders),
+// --seed, --tsteps (JACOBI_2D), --nk/--alpha/--beta (GEMM), --d1d/--q1d (FEM orders),
 // --json <file> (machine-readable results), --tuning-cache <file> (the
 // autotuner's decisions for the "auto" tunings), --graph (graph-replayed rep
 // loops).  --npasses-combiners is the suite's.
@@ -59,6 +59,12 @@ class RunParams {
   std::uint64_t getSeed() const { return seed; }
   std::int64_t getTsteps() const { return tsteps; }
   std::int64_t getNk() const { return nk; }
+  // GEMM scalars from --alpha / --beta; has* false = the suite's (or the
+  // seeded workload's) values.
+  bool hasGemmAlpha() const { return has_alpha; }
+  bool hasGemmBeta() const { return has_beta; }
+  double getGemmAlpha() const { return gemm_alpha; }
+  double getGemmBeta() const { return gemm_beta; }
   int getD1D() const { return d1d; }
   int getQ1D() const { return q1d; }
   int getGPUBlockSize() const { return gpu_block_size; }
@@ -104,6 +110,8 @@ class RunParams {
   std::uint64_t seed = 42;
   std::int64_t tsteps = 0;
   std::int64_t nk = 0;
+  bool has_alpha = false, has_beta = false;
+  double gemm_alpha = 0.0, gemm_beta = 0.0;
   int d1d = 0, q1d = 0;
   int gpu_block_size = 0;
   int items_per_thread = 0;

@@ -20,9 +20,12 @@ POLYBENCH_GEMM::POLYBENCH_GEMM(const RunParams& params) : KernelBase(Polybench_G
   m_ni = static_cast<Index_type>(std::sqrt(double(getTargetProblemSize())) + std::sqrt(2.0) - 1);
   m_nj = m_ni;
   m_nk = params.getNk() > 0 ? params.getNk() : Index_type(double(nk_default) / ni_default * m_ni);
-  // Suite constants (POLYBENCH_GEMM.cpp:35-36); the seeded workload uses 1.5 / 1.2.
+  // Suite constants (POLYBENCH_GEMM.cpp:35-36); the seeded workload uses 1.5 / 1.2;
+  // --alpha / --beta override either.
   m_alpha = params.getInputMode() == RunParams::SeededInputs ? 1.5 : 0.62;
   m_beta = params.getInputMode() == RunParams::SeededInputs ? 1.2 : 1.002;
+  if (params.hasGemmAlpha()) m_alpha = params.getGemmAlpha();
+  if (params.hasGemmBeta()) m_beta = params.getGemmBeta();
   setActualProblemSize(m_ni * m_nj);
   setItsPerRep(m_ni * m_nj);
   setKernelsPerRep(1);

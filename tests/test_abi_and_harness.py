@@ -123,6 +123,21 @@ def test_harness_dryrun_shapes_match_reference_formulas():
     assert gemm[8] == str((1 + 3 * 4096) * 4096 * 4096)               # POLYBENCH_GEMM.cpp:47-48
 
 
+def test_harness_gemm_alpha_beta_flags(orc):
+    """--alpha / --beta set the GEMM scalars (the suite fixes them at
+    POLYBENCH_GEMM.cpp:35-36); non-numbers and non-finite values are bad
+    input.  In the oracle, doubling alpha doubles the checksum exactly (2x
+    is exact through alpha*A, the sums and the weighted checksum), and beta
+    never reaches the result."""
+    assert run_harness("--dryrun", "-k", "GEMM", "--alpha", "2.5", "--beta", "-1").returncode == 0
+    for bad in (["--alpha", "nan"], ["--beta", "x"], ["--alpha"]):
+        r = run_harness("--dryrun", "-k", "GEMM", *bad)
+        assert r.returncode != 0 and "Bad input" in r.stdout
+    base = orc.run("Polybench_GEMM", size=40 * 40)
+    assert orc.run("Polybench_GEMM", size=40 * 40, alpha=2 * 0.62)["checksum"] == 2 * base["checksum"]
+    assert orc.run("Polybench_GEMM", size=40 * 40, beta=5.0)["checksum"] == base["checksum"]
+
+
 def test_harness_without_device_fails_cleanly():
     if abi.device_count() > 0:
         pytest.skip("a device is present")

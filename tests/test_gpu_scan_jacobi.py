@@ -37,6 +37,31 @@ def test_scan_matches_oracle(rpx, orc, rng, n, ipt, bs):
     assert rpx.lib().rpx_scan_scratch_status(s.ptr) == 0
 
 
+@pytest.mark.parametrize("bs,ipt", [(256, 16), (256, 64), (512, 32)])
+@pytest.mark.parametrize("n", [1, 2, 4097, 12289, 1_000_003])
+def test_scan_inclusive(rpx, orc, rng, n, ipt, bs):
+    """The inclusive form: y[i] = exclusive[i] + x[i].  Exact on integer
+    data (against numpy's cumsum) and, on uniform data, within the exclusive
+    scan's bound of the oracle's exclusive prefix plus x[i]."""
+    s, sb = scan_scratch(rpx, n)
+    y = rpx.DeviceBuffer(8 * n)
+    xi = rng.integers(-50, 50, n).astype(np.float64)
+    dx = dev(rpx, xi)
+    rpx.lib().rpx_scan_inclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(bs, ipt), None)
+    assert np.array_equal(y.to_array(count=n), np.cumsum(xi))
+    x = rng.uniform(-1, 1, n)
+    dx = dev(rpx, x)
+    rpx.lib().rpx_scan_inclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(bs, ipt), None)
+    got = y.to_array(count=n)
+    ref = orc.scan(x) + x
+    bound = 1e-15 * np.cumsum(np.abs(x))
+    assert np.all(np.abs(got - ref) <= 8 * np.sqrt(np.arange(1, n + 1)) * bound + 1e-14)
+    # the exclusive scan after an inclusive one on the same scratch
+    rpx.lib().rpx_scan_exclusive_f64(dx.ptr, y.ptr, n, s.ptr, sb, rpx.tuning(bs, ipt), None)
+    assert y.to_array(count=n)[0] == 0.0
+    assert rpx.lib().rpx_scan_scratch_status(s.ptr) == 0
+
+
 def test_scan_exact_on_integers_large(rpx, rng):
     n = 1 << 26
     x = rng.integers(-50, 50, n).astype(np.float64)

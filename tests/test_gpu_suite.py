@@ -24,6 +24,7 @@ CASES = [
     ("Stream_TRIAD", 100_003, []), ("Stream_DOT", 100_003, []),
     ("Algorithm_REDUCE_SUM", 100_003, []), ("Algorithm_SCAN", 100_003, []), ("Algorithm_SORT", 20_011, []),
     ("Polybench_JACOBI_2D", 130 * 130, ["--tsteps", "5"]), ("Polybench_GEMM", 100 * 100, []),
+    ("Polybench_GEMM", 100 * 100, ["--alpha", "-0.5", "--beta", "3"]),
     ("Apps_MASS3DPA", 125 * 300, []), ("Apps_DIFFUSION3DPA", 64 * 300, []),
     ("Apps_MASS3DPA", 216 * 50, ["--d1d", "5", "--q1d", "6"]),
     ("Apps_DIFFUSION3DPA", 125 * 50, ["--d1d", "4"]),
@@ -40,7 +41,7 @@ def test_harness_checksum_matches_base_seq(rpx, orc, kernel, size, extra, mode):
     (run,) = [r for r in res["runs"] if r["kernel"] == kernel]
     kw = dict(size=size, checkrun=3, npasses=2, input_mode=1 if mode == "seeded" else 0)
     for a, b in zip(extra[::2], extra[1::2]):
-        kw[a.lstrip("-")] = int(b)
+        kw[a.lstrip("-")] = float(b) if a in ("--alpha", "--beta") else int(b)
     ref = orc.run(kernel, "Base_Seq", **kw)
     assert run["problem_size"] == ref["problem_size"] and run["reps"] == ref["reps"]
     assert run["bytes_per_rep"] == ref["bytes_per_rep"]
@@ -181,7 +182,7 @@ def test_harness_mixed_tunings_match_base_seq(rpx, orc, kernel, size, extra, mod
     assert run["tuning"] == "mixed"
     kw = dict(size=size, checkrun=2, input_mode=1 if mode == "seeded" else 0)
     for a, b in zip(extra[::2], extra[1::2]):
-        kw[a.lstrip("-")] = int(b)
+        kw[a.lstrip("-")] = float(b) if a in ("--alpha", "--beta") else int(b)
     ref = orc.run(kernel, "Base_Seq", **kw)
     ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
     assert abs(float(run["checksum"] - ref_cs)) <= 1e-12 * max(1.0, abs(float(ref_cs)))
@@ -204,7 +205,7 @@ def test_harness_mixed_tuning_matches_base_seq(rpx, orc, kernel, size, extra, mo
     assert run["tuning"] == "mixed"
     kw = dict(size=size, checkrun=3, npasses=1, input_mode=1 if mode == "seeded" else 0)
     for a, b in zip(extra[::2], extra[1::2]):
-        kw[a.lstrip("-")] = int(b)
+        kw[a.lstrip("-")] = float(b) if a in ("--alpha", "--beta") else int(b)
     ref = orc.run(kernel, "Base_Seq", **kw)
     from fractions import Fraction
     ref_cs = Fraction(ref["checksum_hi"]) + Fraction(ref["checksum_lo"])
