@@ -104,6 +104,8 @@ def config2(rows, reps):
          _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, t512, None), reps), 16 * n)
     _row(rows, "Algorithm_SCAN(ipt 64, cached accesses)", 2,
          _time(lambda: L.rpx_scan_exclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, cached, None), reps), 16 * n)
+    _row(rows, "Algorithm_SCAN(inclusive form, ipt 64)", 2,
+         _time(lambda: L.rpx_scan_inclusive_f64(b.ptr, a.ptr, n, sscr.ptr, ssb, None, None), reps), 16 * n)
     if L.rpx_scan_scratch_status(sscr.ptr) != 0:
         raise RuntimeError("scan look-back wait expired")
     del a, b, c
@@ -182,6 +184,9 @@ def config4(rows, reps, sample_rows=8):
     Cf, Cm = B(8 * n * n), B(8 * n * n)
     alpha = 1.5
     flops = 2 * n ** 3
+
+    def suite_rate(ms):  # secondary rate with the suite's own count (1 + 3nk) ni nj, POLYBENCH_GEMM.cpp:47-48
+        return {"suite_TFLOP/s": round((1 + 3 * n) * n * n / (ms / 1e3) / 1e12, 2)}
     # Ten untimed calls before each timed GEMM: per-call times settle only
     # after several calls (clocks rising under the sustained load).
     ms_f64 = _time(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, 1.2, n, n, n, None, None), reps,
@@ -198,8 +203,9 @@ def config4(rows, reps, sample_rows=8):
     scale = alpha * (np.abs(A[rows_idx]) @ np.abs(Bh))
     C64 = Cf.to_array().reshape(n, n)[rows_idx]
     err_f64 = float(np.max(np.abs(C64 - ref) / scale))
-    _row(rows, "Polybench_GEMM(fp64-mfma)", 4, ms_f64, None, flops, "mfma", FP64_TFLOPS, err_vs_exact=err_f64)
-    _row(rows, "Polybench_GEMM(fp64-naive)", 4, ms_naive, None, flops, "mfma", FP64_TFLOPS)
+    _row(rows, "Polybench_GEMM(fp64-mfma)", 4, ms_f64, None, flops, "mfma", FP64_TFLOPS, err_vs_exact=err_f64,
+         **suite_rate(ms_f64))
+    _row(rows, "Polybench_GEMM(fp64-naive)", 4, ms_naive, None, flops, "mfma", FP64_TFLOPS, **suite_rate(ms_naive))
     # T moduli: T int8 GEMMs; the error bound is 2^(2-b) relative to
     # |alpha| rowmax colmax nk, b from the library (13 moduli: b = 44,
     # 12 moduli: b = 40 at nk = 4096).
@@ -215,7 +221,8 @@ def config4(rows, reps, sample_rows=8):
         _row(rows, f"Polybench_GEMM(mixed-int8-{nmod}mod)", 4, ms_mix, None, flops, "mfma", FP64_TFLOPS,
              err_vs_exact=err_mix, err_bound=2.0 ** (2 - bits), int8_gemms=nmod,
              int8_tops=round(nmod * flops / (ms_mix / 1e3) / 1e12, 1),
-             speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2))
+             speedup_vs_fp64_mfma=round(ms_f64 / ms_mix, 3), speedup_vs_naive=round(ms_naive / ms_mix, 2),
+             **suite_rate(ms_mix))
         del scr
     # The textbook form C = alpha*A*B + beta*C0 (RPX_TUNE_GEMM_BETA), a
     # separately labelled output: errors from one call on C0, against
@@ -233,7 +240,7 @@ def config4(rows, reps, sample_rows=8):
     err_tb = float(np.max(np.abs(Cf.to_array().reshape(n, n)[rows_idx] - ref_tb) / scale_tb))
     ms_tb = _time(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, tb, None), reps, warmup=10)
     _row(rows, "Polybench_GEMM(fp64-mfma, textbook alpha*AB+beta*C)", 4, ms_tb, None, flops, "mfma", FP64_TFLOPS,
-         err_vs_exact=err_tb)
+         err_vs_exact=err_tb, **suite_rate(ms_tb))
     sb = L.rpx_gemm_mixed_scratch_bytes(n, n, n, 0)
     scr = B(sb)
     L.rpx_memcpy_d2d(Cm.ptr, C0.ptr, 8 * n * n, None)
@@ -242,7 +249,7 @@ def config4(rows, reps, sample_rows=8):
     ms_tbm = _time(lambda: L.rpx_gemm_mixed_tuned_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, 0, tb, scr.ptr,
                                                       sb, None), reps, warmup=10)
     _row(rows, "Polybench_GEMM(mixed-int8-13mod, textbook alpha*AB+beta*C)", 4, ms_tbm, None, flops, "mfma",
-         FP64_TFLOPS, err_vs_exact=err_tbm, speedup_vs_fp64_mfma=round(ms_tb / ms_tbm, 3))
+         FP64_TFLOPS, err_vs_exact=err_tbm, speedup_vs_fp64_mfma=round(ms_tb / ms_tbm, 3), **suite_rate(ms_tbm))
     del scr, C0
     del dA, dB, Cf, Cm
     mass_mixed(rows, reps)
