@@ -185,7 +185,9 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
 
   // Exclusive scan inside the wavefront, row by row: lane pair sums, a
   // 64-lane scan on DPP moves, and a running carry of the row totals.  The
-  // inclusive form (INCL) adds each element to its exclusive value.
+  // inclusive form (INCL) adds each element to its exclusive value; at 32
+  // rows its instance holds 32 more values than the registers allow (they
+  // go to AGPRs) and runs 0.88 against 0.72 ms at 2^28.
   double carry = 0.0;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {

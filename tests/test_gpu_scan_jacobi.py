@@ -40,8 +40,8 @@ def test_scan_matches_oracle(rpx, orc, rng, n, ipt, bs):
 @pytest.mark.parametrize("bs,ipt", [(256, 16), (256, 64), (512, 32)])
 @pytest.mark.parametrize("n", [1, 2, 4097, 12289, 1_000_003])
 def test_scan_inclusive(rpx, orc, rng, n, ipt, bs):
-    """The inclusive form: y[i] = exclusive[i] + x[i].  Exact on integer
-    data (against numpy's cumsum) and, on uniform data, within the exclusive
+    """The inclusive form y[i] = x[0] + ... + x[i].  Exact on integer data
+    (against numpy's cumsum) and, on uniform data, within the exclusive
     scan's bound of the oracle's exclusive prefix plus x[i]."""
     s, sb = scan_scratch(rpx, n)
     y = rpx.DeviceBuffer(8 * n)
