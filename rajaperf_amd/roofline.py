@@ -365,10 +365,18 @@ def config5(rows, reps):
 
 
 def graph_pass(rows, reps):
-    """One pass of every kernel at randomly drawn sizes, run eagerly and as
-    one captured HIP graph."""
-    from .suite_pass import SuitePass, draw_sizes
-    sizes = draw_sizes(np.random.default_rng(SEED))
+    """One pass of every kernel run eagerly and as one captured HIP graph:
+    at randomly drawn sizes, and at the smallest sizes, where the pass is
+    bound by launches rather than by the kernels."""
+    from .suite_pass import draw_sizes
+    _graph_pass_at(rows, reps, draw_sizes(np.random.default_rng(SEED)), "random sizes")
+    small = {"n_stream": 4096, "n_scan": 4096, "n_sort": 1024, "gemm": (64, 64, 64), "jacobi": (130, 2),
+             "mass": (64, 3), "diffusion": (64, 3)}
+    _graph_pass_at(rows, reps, small, "smallest sizes, launch-bound")
+
+
+def _graph_pass_at(rows, reps, sizes, label):
+    from .suite_pass import SuitePass
     p = SuitePass(sizes)
     st = abi.Stream()
     with abi.Graph(st) as graph:
@@ -388,8 +396,8 @@ def graph_pass(rows, reps):
         return e0.elapsed_ms(e1) / reps
     ms_eager = timed(lambda: p.run(st))
     ms_graph = timed(graph.launch)
-    _row(rows, "Suite pass of all kernels (random sizes), one HIP graph", 5, ms_graph, p.bytes_per_pass(),
-         eager_ms=round(ms_eager, 5), sizes=str(sizes).replace(" ", ""))
+    _row(rows, f"Suite pass of all kernels ({label}), one HIP graph", 5, ms_graph, p.bytes_per_pass(),
+         eager_ms=round(ms_eager, 5), graph_speedup=round(ms_eager / ms_graph, 2), sizes=str(sizes).replace(" ", ""))
 
 
 def mass_batches(rows, reps, nbatch=64):
