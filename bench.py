@@ -367,6 +367,33 @@ def mixed_vs_fp64(reps: int = 5) -> dict:
     scale = alpha * (np.abs(A) @ np.abs(Bh))
     err_mix = float(np.max(np.abs(Cm.to_array().reshape(n, n)[rows] - ref) / scale))
     err_f64 = float(np.max(np.abs(Cf.to_array().reshape(n, n)[rows] - ref) / scale))
+    # The textbook form alpha*A*B + beta*C0 (RPX_TUNE_GEMM_BETA), reported
+    # beside the suite's form: errors from one call on C0, times over
+    # repeated calls after the same warm-up.
+    beta = 1.2
+    C0 = B(8 * n * n)
+    L.rpx_fill_uniform_f64(C0.ptr, n * n, SEED, 4, -1.0, 1.0, None)
+    c0 = C0.to_array().reshape(n, n)[rows]
+    ref_tb = ref + np.longdouble(beta) * c0.astype(np.longdouble)
+    scale_tb = scale + abs(beta) * np.abs(c0)
+    tb = abi.tuning(flags=abi.RPX_TUNE_GEMM_BETA)
+    tb_err = {}
+    for name, C, call in (("fp64_mfma", Cf, lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, tb,
+                                                                    None)),
+                          ("mixed", Cm, lambda: L.rpx_gemm_mixed_tuned_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, beta, n, n,
+                                                                           n, 0, tb, scr.ptr, sb, None))):
+        L.rpx_memcpy_d2d(C.ptr, C0.ptr, 8 * n * n, None)
+        call()
+        tb_err[name] = float(np.max(np.abs(C.to_array().reshape(n, n)[rows] - ref_tb) / scale_tb))
+    ms_tb_f64 = _timed(lambda: L.rpx_gemm_f64(Cf.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, tb, None), reps,
+                       warmup=10)
+    ms_tb_mix = _timed(lambda: L.rpx_gemm_mixed_tuned_f64(Cm.ptr, dA.ptr, dB.ptr, alpha, beta, n, n, n, 0, tb,
+                                                          scr.ptr, sb, None), reps, warmup=10)
+    textbook = {"form": "alpha*A*B + beta*C, beta 1.2, C0 U[-1,1) (the suite's own form ignores beta)",
+                "fp64_mfma_ms": round(ms_tb_f64, 4), "mixed_ms": round(ms_tb_mix, 4),
+                "speedup_vs_fp64_mfma": round(ms_tb_f64 / ms_tb_mix, 3),
+                "max_err_rel_mixed": tb_err["mixed"], "max_err_rel_fp64": tb_err["fp64_mfma"],
+                "error_reference": "80-bit host on 8 sampled rows, relative to |alpha|(|A||B|) + |beta C0|"}
     return {"kernel": "Polybench_GEMM 4096^3, alpha 1.5, U[-1,1)", "moduli": 13, "operand_bits": bits,
             "fp64_mfma_ms": round(ms_f64, 4),
             "plain_fp64_ms": round(ms_plain, 3), "mixed_ms": round(ms_mix, 4),
@@ -374,7 +401,7 @@ def mixed_vs_fp64(reps: int = 5) -> dict:
             "speedup_vs_fp64_mfma": round(ms_f64 / ms_mix, 3), "speedup_vs_plain_fp64": round(ms_plain / ms_mix, 2),
             "speedup_vs_vendor_dgemm": None if ms_vendor is None else round(ms_vendor / ms_mix, 3),
             "max_err_rel_abs_ab_mixed": err_mix, "max_err_rel_abs_ab_fp64": err_f64,
-            "error_reference": "80-bit host sums on 8 sampled rows"}
+            "error_reference": "80-bit host sums on 8 sampled rows", "textbook_beta": textbook}
 
 
 def mixed_mass_vs_fp64(reps: int = 10) -> dict:
