@@ -136,7 +136,7 @@ MixedLayout mixed_layout(int64_t ni, int64_t nj, int64_t nk, int T) {
   MixedLayout l;
   l.nip = round_up(ni, kTM);
   l.njp = round_up(nj, kTN);
-  l.nkp = round_up(nk > 0 ? nk : 1, kTK);
+  l.nkp = round_up(nk > 0 ? nk : 1, 2 * kTK);  // the K loop takes two stages per step
   l.a_off = 0;
   l.b_off = align256(size_t(T) * l.nip * l.nkp);
   l.ea_off = l.b_off + align256(size_t(T) * l.njp * l.nkp);
@@ -325,102 +325,116 @@ static_assert(kChunks == 16 || kChunks == 8, "swizzle covers 8 or 16 chunks per 
 // modulo m_t and stores the byte.  The tile size is set by L2 bandwidth: a
 // stage brings 128 bytes of K for 256 rows of A and of B (64 KiB) for
 // 8.4 M MACs, 128 MACs per byte, where a 128x128 tile's 64 left the MFMAs
-// waiting on L2.  Two stages of LDS (128 KiB), one workgroup per CU; the
-// next stage's global loads are in flight in registers while one computes.
+// waiting on L2.  Two stages of LDS (128 KiB), one workgroup per CU.
 constexpr int kThreads = 512;
 constexpr int kWaveM = 64, kWaveN = 128;
 constexpr int kFm = kWaveM / 16, kFn = kWaveN / 16;  // 16x16 tiles per wavefront
 constexpr int kFrags = (kThreads / kWave) * kFm * kFn;  // accumulator fragments per workgroup tile
-constexpr int kLoads = kTM * kChunks / kThreads;       // 16-byte chunks per work-item per operand
-static_assert(kTM == kTN, "one loader mapping for both operands");
+static_assert(kTM == kTN && kTM == 256 && kChunks == 8, "the stage loader covers 256 rows of 8 chunks");
 
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void global_void;
+
+// Stages go from global memory straight into LDS (global_load_lds, 16
+// bytes per lane: each instruction fills 64 consecutive slots, i.e. 8
+// swizzled rows, so lane l loads the logical chunk that swz() puts in slot
+// l), one stage ahead of the one being computed.  All stages of all moduli
+// form one sequence g = t * ksteps + st, so the next modulus's first stage
+// is loaded during the current modulus's last.  The two stage buffers are
+// separate arrays and the loop is unrolled by two, so every LDS access
+// names its buffer: the compiler then sees that the loads in flight cannot
+// write what the fragment reads read, and adds no wait for them; the wait
+// for a stage is explicit (vmcnt(0), then the barrier that also frees the
+// other buffer).  Against register-staged stages (loads into registers,
+// then LDS stores a step later) this saves 32 VGPRs and the LDS store
+// instructions: 1.05-1.06 against 1.08-1.10 ms per 4096^3 call.
 __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restrict__ R,
                                                               const signed char* __restrict__ As,
                                                               const signed char* __restrict__ Bs, int64_t nip,
                                                               int64_t njp, int64_t nkp, int tiles_n,
                                                               const CrtParams P) {
-  __shared__ v4i Al[2][kTM * kChunks];
-  __shared__ v4i Bl[2][kTN * kChunks];
+  __shared__ v4i A0[kTM * kChunks], A1[kTM * kChunks];
+  __shared__ v4i B0[kTN * kChunks], B1[kTN * kChunks];
   int64_t tm, tn;
   tile_of(blockIdx.x, gridDim.x, int(nip / kTM), tiles_n, tm, tn);
   const int64_t m0 = tm * kTM, n0 = tn * kTN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * kWaveM, wn = (wave & 1) * kWaveN;
-  const int ksteps = int(nkp / kTK);
-  const int total = P.T * ksteps;  // stages over all moduli
-  // Loader mapping: work-item tid moves chunk lc of rows lr + kRowStep*u, u < kLoads.
-  constexpr int kRowStep = kThreads / kChunks;
-  const int lc = tid % kChunks, lr = tid / kChunks;
+  const int ksteps = int(nkp / kTK);  // even: nkp is a multiple of 2 kTK
+  const int total = P.T * ksteps;     // stages over all moduli
   const int fr = lane & 15, fk = lane >> 4;  // MFMA operand row and k quarter
   // This wavefront's fragments in the residue array: [tile][wave][i][j][lane].
   const int64_t frag0 = ((tm * tiles_n + tn) * (kThreads / kWave) + wave) * (kFm * kFn);
   const int64_t per_mod = nip * njp / 4;  // 32-bit words per modulus
-  const signed char* a0 = As + (m0 + lr) * nkp + 16 * lc;
-  const signed char* b0 = Bs + (n0 + lr) * nkp + 16 * lc;
+  // Loader: instruction u (0..3) of wavefront w fills slots 64 (4w + u) ..
+  // + 63: row drow0 + 8u (drow0 = 32w + lane/8), physical chunk lane%8,
+  // which holds logical chunk (lane%8) ^ ((row >> 1) & 7).
+  const int drow0 = 32 * wave + (lane >> 3);
+  int dchunk[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) dchunk[u] = (lane & 7) ^ (((drow0 + 8 * u) >> 1) & 7);
+  const signed char* a0 = As + (m0 + drow0) * nkp;
+  const signed char* b0 = Bs + (n0 + drow0) * nkp;
   const int64_t a_mod = nip * nkp, b_mod = njp * nkp;
-
-  v4i ga[kLoads], gb[kLoads];
-  auto gload = [&](int g) {  // stage g: modulus g / ksteps, K step g % ksteps
+  auto load_stage = [&](int g, v4i* Ad, v4i* Bd) {  // stage g: modulus g / ksteps, K step g % ksteps
     const int tt = g / ksteps, st = g - tt * ksteps;
     const signed char* at = a0 + tt * a_mod + st * kTK;
     const signed char* bt = b0 + tt * b_mod + st * kTK;
 #pragma unroll
-    for (int u = 0; u < kLoads; ++u) {
-      ga[u] = *reinterpret_cast<const v4i*>(at + int64_t(kRowStep * u) * nkp);
-      gb[u] = *reinterpret_cast<const v4i*>(bt + int64_t(kRowStep * u) * nkp);
+    for (int u = 0; u < 4; ++u) {
+      __builtin_amdgcn_global_load_lds((global_void*)(at + int64_t(8 * u) * nkp + 16 * dchunk[u]),
+                                       (lds_void*)(Ad + 64 * (4 * wave + u)), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((global_void*)(bt + int64_t(8 * u) * nkp + 16 * dchunk[u]),
+                                       (lds_void*)(Bd + 64 * (4 * wave + u)), 16, 0, 0);
     }
   };
-  auto lstore = [&](int buf) {
+  v4i acc[kFm][kFn];
+  auto compute = [&](const v4i* Ar, const v4i* Br) {
+    // Raised wave priority over the MFMA block: with two wavefronts per
+    // SIMD the arbiter then keeps feeding the matrix pipe from one while
+    // the other issues its loads.
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-    for (int u = 0; u < kLoads; ++u) {
-      Al[buf][swz(lr + kRowStep * u, lc)] = ga[u];
-      Bl[buf][swz(lr + kRowStep * u, lc)] = gb[u];
+    for (int kk = 0; kk < kChunks / 4; ++kk) {
+      const int c = kk * 4 + fk;
+      v4i af[kFm], bf[kFn];
+#pragma unroll
+      for (int i = 0; i < kFm; ++i) af[i] = Ar[swz(wm + i * 16 + fr, c)];
+#pragma unroll
+      for (int j = 0; j < kFn; ++j) bf[j] = Br[swz(wn + j * 16 + fr, c)];
+#pragma unroll
+      for (int i = 0; i < kFm; ++i)
+#pragma unroll
+        for (int j = 0; j < kFn; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
+    __builtin_amdgcn_s_setprio(0);
+  };
+  // This wavefront's stage loads have landed (vmcnt(0)); after the barrier
+  // every wavefront's have, and every wavefront has finished reading the
+  // other buffer (its reads fed the MFMAs issued before the barrier).
+  auto stage_ready = [&]() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
   };
 
-  // The stages of all moduli form one pipeline: stage g + 1 is in flight in
-  // registers while stage g computes, so the first stage of the next
-  // modulus is loaded and stored like any other, and the load of the one
-  // after it is issued before the epilogue.
-  gload(0);
-  lstore(0);
-  __syncthreads();
-  if (total > 1) gload(1);
+  load_stage(0, A0, B0);
   int g = 0;
   for (int t = 0; t < P.T; ++t) {
-    v4i acc[kFm][kFn];
 #pragma unroll
     for (int i = 0; i < kFm; ++i)
 #pragma unroll
       for (int j = 0; j < kFn; ++j) acc[i][j] = v4i{0, 0, 0, 0};
-    for (int st = 0; st < ksteps; ++st, ++g) {
-      const int buf = g & 1;
-      // Raised wave priority over the MFMA block: with two wavefronts per
-      // SIMD the arbiter then keeps feeding the matrix pipe from one while
-      // the other issues its loads and LDS stores.
-      __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-      for (int kk = 0; kk < kChunks / 4; ++kk) {
-        const int c = kk * 4 + fk;
-        v4i af[kFm], bf[kFn];
-#pragma unroll
-        for (int i = 0; i < kFm; ++i) af[i] = Al[buf][swz(wm + i * 16 + fr, c)];
-#pragma unroll
-        for (int j = 0; j < kFn; ++j) bf[j] = Bl[buf][swz(wn + j * 16 + fr, c)];
-#pragma unroll
-        for (int i = 0; i < kFm; ++i)
-#pragma unroll
-          for (int j = 0; j < kFn; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-      }
-      __builtin_amdgcn_s_setprio(0);
-      if (g + 1 < total) {
-        lstore(buf ^ 1);  // buffer buf ^ 1 was last read before the previous barrier
-        __syncthreads();
-      }
-      if (st + 1 < ksteps && g + 2 < total) gload(g + 2);
+    for (int st = 0; st < ksteps; st += 2, g += 2) {
+      stage_ready();  // stage g in buffer 0
+      if (g + 1 < total) load_stage(g + 1, A1, B1);
+      compute(A0, B0);
+      stage_ready();  // stage g + 1 in buffer 1
+      if (g + 2 < total) load_stage(g + 2, A0, B0);
+      compute(A1, B1);
     }
-    if (g + 1 < total) gload(g + 1);  // g is the next modulus's first stage, already in LDS
     // d_t = the sum's symmetric residue modulo m_t (|sum| < 2^31, see
     // sym_mod); the four rows of a lane's accumulator go to one 32-bit word.
     const double m = P.m[t], inv = P.inv[t];
