@@ -407,6 +407,10 @@ __global__ __launch_bounds__(kThreads) void ozaki_gemm_kernel(unsigned* __restri
 #pragma unroll
         for (int j = 0; j < kFn; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+      // The backend's LDS-read/MFMA interleave for small GEMM loops: 1.056-1.063
+      // against 1.085-1.093 ms per 4096^3 call, alternating runs (an explicit
+      // one LDS read per two MFMAs was slower, 1.10 ms).
+      __builtin_amdgcn_iglp_opt(0);
     }
     __builtin_amdgcn_s_setprio(0);
   };
