@@ -217,8 +217,8 @@ int rpx_scan_scratch_status(const void* scratch);
 int rpx_scan_exclusive_f64(const double* x, double* y, int64_t n,
                            void* scratch, size_t scratch_bytes,
                            const rpx_tuning* tune, void* stream);
-/* Inclusive form, y[i] = x[0] + ... + x[i] (each element added to its
- * exclusive value), an extra output beside the suite's exclusive SCAN
+/* Inclusive form, y[i] = x[0] + ... + x[i] (within a wavefront, the
+ * exclusive prefix of element i+1), an extra output beside the suite's exclusive SCAN
  * (src/algorithm/SCAN.hpp:27-32 defines only the exclusive one).  Same
  * scratch, tunings and conventions as rpx_scan_exclusive_f64. */
 int rpx_scan_inclusive_f64(const double* x, double* y, int64_t n,
@@ -226,11 +226,14 @@ int rpx_scan_inclusive_f64(const double* x, double* y, int64_t n,
                            const rpx_tuning* tune, void* stream);
 /* Segmented form: segment s covers [offsets[s], offsets[s+1]) and is scanned
  * independently; `offsets_host` is a host array of nseg+1 ascending values.
- * All segments run in one launch (tiles of 16384 doubles, the look-back of
- * a tile stops at its segment's first tile).  The call copies the segment
- * tables into `scratch` on `stream` from the library's pinned staging
- * memory, without a host synchronisation (a staging slot is reused once
- * its copy has completed); the scratch needs no initialisation. */
+ * All segments run in one launch (tiles of 16384 doubles counted from the
+ * segment's start rounded down to an even element, so a segment's first
+ * tile may be one element short; the look-back of a tile stops at its
+ * segment's first tile).  The call copies the offsets and each segment's
+ * first tile into `scratch` on `stream` in one copy from the library's
+ * pinned staging memory, without a host synchronisation (a staging slot is
+ * reused once its copy has completed); the scratch needs no
+ * initialisation. */
 size_t rpx_scan_segmented_scratch_bytes(const int64_t* offsets_host,
                                         int64_t nseg);
 int rpx_scan_exclusive_segmented_f64(const double* x, double* y,

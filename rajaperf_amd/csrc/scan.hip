@@ -106,19 +106,26 @@ __device__ __forceinline__ double wave_inclusive_scan_dpp(double v) {
 
 // Lane l receives lane l-1's value, lane 0 receives 0 (wave_shr:1).
 __device__ __forceinline__ double wave_shift_up_dpp(double v) { return dpp_move<0x138, 0xF, true>(v); }
+// Lane l receives lane l+1's value, lane 63 receives 0 (wave_shl:1).
+__device__ __forceinline__ double wave_shift_down_dpp(double v) { return dpp_move<0x130, 0xF, true>(v); }
 
-// Lane 63's value, wavefront-uniform.
-__device__ __forceinline__ double last_lane(double v) {
+// Lane L's value, wavefront-uniform.
+template <int L>
+__device__ __forceinline__ double lane_value(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane(int(b), kWave - 1);
-  const int hi = __builtin_amdgcn_readlane(int(b >> 32), kWave - 1);
+  const int lo = __builtin_amdgcn_readlane(int(b), L);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), L);
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(unsigned(hi)) << 32) |
                                                      unsigned(lo)));
 }
+__device__ __forceinline__ double last_lane(double v) { return lane_value<kWave - 1>(v); }
 
 // Segment tables of a segmented scan: segment s covers [off[s], off[s+1])
 // and owns the global tiles [tile0[s], tile0[s+1]); tile t belongs to
-// segment tile_seg[t].
+// segment tile_seg[t].  A segment's tiles start at even elements (the
+// segment's start rounded down to even, then whole tiles), so every tile
+// after its first reads and writes 16-byte rows; the first tile skips the
+// element before an odd start.
 struct SegTables {
   const int64_t* off;
   const int64_t* tile0;
@@ -148,22 +155,24 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
   if (!SEG && tid == 0) s_epoch = __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
   const int64_t t = blockIdx.x;
-  int64_t tbeg, tend, first;  // the tile's elements [tbeg, tend); its segment's first tile
+  // The tile spans [tal, tal + kTileN) from an even tal and owns the
+  // elements [lo, tend) of it; first is its segment's first tile.
+  int64_t tal, lo, tend, first;
   if (SEG) {
     const int sg = seg.tile_seg[t];
     first = seg.tile0[sg];
-    tbeg = seg.off[sg] + (t - first) * kTileN;
-    const int64_t send = seg.off[sg + 1];
-    tend = (tbeg + kTileN < send) ? tbeg + kTileN : send;
+    const int64_t sbeg = seg.off[sg], send = seg.off[sg + 1];
+    tal = (sbeg & ~int64_t(1)) + (t - first) * kTileN;
+    lo = sbeg > tal ? sbeg : tal;
+    tend = (tal + kTileN < send) ? tal + kTileN : send;
   } else {
     first = 0;
-    tbeg = t * kTileN;
-    tend = (tbeg + kTileN < n) ? tbeg + kTileN : n;
+    tal = lo = t * kTileN;
+    tend = (tal + kTileN < n) ? tal + kTileN : n;
   }
-  const int64_t base = tbeg + int64_t(wave) * kChunk + 2 * lane;
-  // x and y are 16-byte aligned (vec), so rows are too when the tile starts
-  // on an even element.
-  const bool full = vec && tend - tbeg == kTileN && (tbeg & 1) == 0;
+  const int64_t base = tal + int64_t(wave) * kChunk + 2 * lane;
+  // x and y are 16-byte aligned (vec) and tal is even, so the rows are too.
+  const bool full = vec && lo == tal && tend - tal == kTileN;
   n = tend;  // bounds of the partial-tile paths below
 
   // Each wavefront owns a contiguous chunk, read as ROWS coalesced 1 KiB rows.
@@ -178,16 +187,16 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       const int64_t i0 = base + r * kRowLen;
-      v[r].x = (i0 < n) ? x[i0] : 0.0;
-      v[r].y = (i0 + 1 < n) ? x[i0 + 1] : 0.0;
+      v[r].x = (i0 >= lo && i0 < n) ? x[i0] : 0.0;
+      v[r].y = (i0 + 1 >= lo && i0 + 1 < n) ? x[i0 + 1] : 0.0;
     }
   }
 
   // Exclusive scan inside the wavefront, row by row: lane pair sums, a
   // 64-lane scan on DPP moves, and a running carry of the row totals.  The
-  // inclusive form (INCL) adds each element to its exclusive value; at 32
-  // rows its instance holds 32 more values than the registers allow (they
-  // go to AGPRs) and runs 0.88 against 0.72 ms at 2^28.
+  // pair sum replaces the pair's second element, so each row keeps two
+  // values; the inclusive form (INCL) is built from these exclusive values
+  // when it is stored.
   double carry = 0.0;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
@@ -196,11 +205,7 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
     const double exc = wave_shift_up_dpp(inc);
     const double row_total = last_lane(inc);
     const double e0 = carry + exc;
-    const double e1 = e0 + v[r].x;
-    if constexpr (INCL)
-      v[r] = v2d{e1, e1 + v[r].y};
-    else
-      v[r] = v2d{e0, e1};
+    v[r] = v2d{e0, e0 + v[r].x};
     carry += row_total;
   }
   if (lane == 0) wave_tot[wave] = carry;
@@ -275,6 +280,19 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
   __syncthreads();
 
   const double off = s_prefix + wave_off;
+  // The inclusive value of an element is the exclusive value of the element
+  // after it: for lane l's pair, its own second value and lane l+1's first
+  // (for lane 63 the next row's first, after the last row the wavefront's
+  // total).  Element n-1 of a partial tile takes the same rule, since the
+  // padding beyond n is zero.
+  if constexpr (INCL) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const double next = (r + 1 < ROWS) ? lane_value<0>(v[r + 1].x) : carry;
+      const double shifted = wave_shift_down_dpp(v[r].x);
+      v[r] = v2d{v[r].y, lane == kWave - 1 ? next : shifted};
+    }
+  }
   if (full) {
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -287,8 +305,8 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       const int64_t i0 = base + r * kRowLen;
-      if (i0 < n) y[i0] = off + v[r].x;
-      if (i0 + 1 < n) y[i0 + 1] = off + v[r].y;
+      if (i0 >= lo && i0 < n) y[i0] = off + v[r].x;
+      if (i0 + 1 >= lo && i0 + 1 < n) y[i0 + 1] = off + v[r].y;
     }
   }
 }
@@ -296,6 +314,28 @@ __global__ __launch_bounds__(BS) void scan_kernel(const double* x, double* y, in
 __global__ void zero_kernel(unsigned* p, int64_t nwords) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nwords; i += stride) p[i] = 0u;
+}
+
+// Before a segmented scan: clears the flags and fills tile_seg from tile0
+// (the last segment whose first tile is <= t owns tile t; empty segments
+// share their first tile with the next segment and so are never chosen).
+__global__ __launch_bounds__(256) void seg_prep_kernel(unsigned* __restrict__ flag_words, int64_t nwords,
+                                                       const int64_t* __restrict__ tile0, int nseg,
+                                                       int* __restrict__ tile_seg, int64_t ntiles) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t end = nwords > ntiles ? nwords : ntiles;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < end; i += stride) {
+    if (i < nwords) flag_words[i] = 0u;
+    if (i < ntiles) {
+      int a = 0, b = nseg - 1;
+      while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (tile0[mid] <= i) a = mid;
+        else b = mid - 1;
+      }
+      tile_seg[i] = a;
+    }
+  }
 }
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -356,30 +396,33 @@ int scan_one(const double* x, double* y, int64_t n, int ipt, int bs, bool nt, vo
 }
 
 // Segmented scan: every tile of every segment in one launch (tiles of
-// 256 x 64 doubles; a segment's last tile may be partial).  Scratch: the
-// single-array layout for the total tile count, then the segment tables.
+// 256 x 64 doubles from the segment's even-rounded start; a segment's first
+// and last tiles may be partial).  Scratch: the single-array layout for the
+// total tile count, then off[nseg+1] and tile0[nseg+1] (one upload), then
+// tile_seg[ntiles] (filled on the device).
 constexpr int64_t kSegTile = int64_t(kBS) * kDefaultIPT;
+
+inline int64_t seg_tiles(int64_t sbeg, int64_t send) {
+  return send > sbeg ? ceil_div(send - (sbeg & ~int64_t(1)), kSegTile) : 0;
+}
 
 struct SegLayout {
   Layout l;
   int64_t ntiles;
-  size_t tile_seg_off, off_off, tile0_off, total;
+  size_t tab_off, tile_seg_off, total;
 };
 SegLayout seg_layout(const int64_t* offsets_host, int64_t nseg) {
   SegLayout g;
   g.ntiles = 0;
-  for (int64_t s = 0; s < nseg; ++s) g.ntiles += ceil_div(offsets_host[s + 1] - offsets_host[s], kSegTile);
+  for (int64_t s = 0; s < nseg; ++s) g.ntiles += seg_tiles(offsets_host[s], offsets_host[s + 1]);
   g.l = layout_for(g.ntiles > 0 ? g.ntiles : 1);
-  g.tile_seg_off = g.l.total;
-  g.off_off = g.tile_seg_off + align256(size_t(g.ntiles) * sizeof(int));
-  g.tile0_off = g.off_off + align256(size_t(nseg + 1) * sizeof(int64_t));
-  g.total = g.tile0_off + align256(size_t(nseg + 1) * sizeof(int64_t));
+  g.tab_off = g.l.total;
+  g.tile_seg_off = g.tab_off + align256(size_t(2 * (nseg + 1)) * sizeof(int64_t));
+  g.total = g.tile_seg_off + align256(size_t(g.ntiles) * sizeof(int));
   return g;
 }
 
 }  // namespace
-
-extern "C" {}  // namespace
 
 extern "C" {
 
@@ -425,29 +468,25 @@ int rpx_scan_exclusive_segmented_f64(const double* x, double* y, const int64_t* 
   if (g.ntiles > INT32_MAX) return RPX_ERR_UNSUPPORTED;
   hipStream_t st = as_stream(stream);
   char* base = reinterpret_cast<char*>(scratch);
-  // Tables: each tile's segment and each segment's first tile.
-  std::vector<int> tile_seg(size_t(g.ntiles));
-  std::vector<int64_t> tile0(size_t(nseg + 1));
+  // Tables: the offsets, then each segment's first tile (one copy).
+  std::vector<int64_t> tab(size_t(2 * (nseg + 1)));
   int64_t t = 0;
-  for (int64_t s = 0; s < nseg; ++s) {
-    tile0[s] = t;
-    const int64_t k = ceil_div(offsets_host[s + 1] - offsets_host[s], kSegTile);
-    for (int64_t i = 0; i < k; ++i) tile_seg[size_t(t + i)] = int(s);
-    t += k;
+  for (int64_t s = 0; s <= nseg; ++s) {
+    tab[size_t(s)] = offsets_host[s];
+    tab[size_t(nseg + 1 + s)] = t;
+    if (s < nseg) t += seg_tiles(offsets_host[s], offsets_host[s + 1]);
   }
-  tile0[nseg] = t;
+  auto* d_off = reinterpret_cast<int64_t*>(base + g.tab_off);
+  auto* d_tile0 = d_off + (nseg + 1);
   auto* d_tile_seg = reinterpret_cast<int*>(base + g.tile_seg_off);
-  auto* d_off = reinterpret_cast<int64_t*>(base + g.off_off);
-  auto* d_tile0 = reinterpret_cast<int64_t*>(base + g.tile0_off);
-  const HostChunk tables[3] = {{d_tile_seg, tile_seg.data(), tile_seg.size() * sizeof(int)},
-                               {d_off, offsets_host, size_t(nseg + 1) * sizeof(int64_t)},
-                               {d_tile0, tile0.data(), tile0.size() * sizeof(int64_t)}};
-  if (const int rc = upload_async(tables, 3, st); rc != RPX_SUCCESS) return rc;
+  const HostChunk table{d_off, tab.data(), tab.size() * sizeof(int64_t)};
+  if (const int rc = upload_async(&table, 1, st); rc != RPX_SUCCESS) return rc;
   // Flags start at zero for every call (no epochs across segments).
   auto* flags = reinterpret_cast<unsigned long long*>(base + g.l.flags_off);
   const int64_t words = g.ntiles * 2;
-  zero_kernel<<<dim3(unsigned(ceil_div(words, 256) < 4096 ? ceil_div(words, 256) : 4096)), dim3(256), 0, st>>>(
-      reinterpret_cast<unsigned*>(flags), words);
+  const int64_t work = words > g.ntiles ? words : g.ntiles;
+  seg_prep_kernel<<<dim3(unsigned(ceil_div(work, 256) < 4096 ? ceil_div(work, 256) : 4096)), dim3(256), 0, st>>>(
+      reinterpret_cast<unsigned*>(flags), words, d_tile0, int(nseg), d_tile_seg, g.ntiles);
   if (const int rc = launch_status(); rc != RPX_SUCCESS) return rc;
   const SegTables seg{d_off, d_tile0, d_tile_seg};
   const int vec = aligned16(x) && aligned16(y);

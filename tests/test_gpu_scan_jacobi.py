@@ -156,6 +156,30 @@ def test_scan_segmented_ragged_one_launch(rpx, orc, seed):
         L.rpx_scan_exclusive_segmented_f64(dx.ptr, y.ptr, o, len(offs) - 1, scr.ptr, sb - 256, None)
 
 
+@pytest.mark.parametrize("start", [1, 2])
+def test_scan_segmented_tile_edges(rpx, orc, rng, start):
+    """Segment lengths around the 16384-element tile at odd and even starts
+    (a segment's tiles are counted from its even-rounded start), with one
+    element before the first segment and one after the last that the call
+    must leave untouched."""
+    T = 16384
+    lens = [1, T - 1, T, T + 1, 2 * T - 1, 2 * T, 2 * T + 1, 1, 2, 0, T - 2, 3, T + 2]
+    offs = np.concatenate([[start], start + np.cumsum(lens)]).tolist()
+    n = offs[-1] + 1
+    x = rng.integers(-50, 50, n).astype(np.float64)
+    dx, y = dev(rpx, x), dev(rpx, np.full(n, 7.5))
+    o = rpx.offsets_array(offs)
+    L = rpx.lib()
+    sb = L.rpx_scan_segmented_scratch_bytes(o, len(offs) - 1)
+    scr = rpx.DeviceBuffer(sb)
+    L.rpx_scan_exclusive_segmented_f64(dx.ptr, y.ptr, o, len(offs) - 1, scr.ptr, sb, None)
+    got = y.to_array()
+    for b, e in zip(offs[:-1], offs[1:]):
+        assert np.array_equal(got[b:e], orc.scan(x[b:e])), (b, e)
+    assert np.all(got[:start] == 7.5) and got[-1] == 7.5
+    assert L.rpx_scan_scratch_status(scr.ptr) == 0
+
+
 @pytest.mark.parametrize("N", [3, 4, 5, 8, 33, 34, 130, 514, 1026, 2050])
 @pytest.mark.parametrize("tsteps", [1, 3])
 def test_jacobi_bit_exact(rpx, orc, rng, N, tsteps):
