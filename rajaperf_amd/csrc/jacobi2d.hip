@@ -147,7 +147,8 @@ __global__ __launch_bounds__(256) void jacobi_scalar_kernel(double* __restrict__
 // wavefront index is made uniform with readfirstlane) and the wavefronts
 // that touch the grid boundary take a separate path.  Measured on MI355X at
 // N = 8192, a launch of depth 3 moves the ideal 1.08 GB (read A, write the
-// new A); R = 16 rows per wavefront is the fastest tile.
+// new A); R = 16 rows per wavefront is the fastest tile of this kernel, and
+// the streaming kernel below (chunks of 99 rows) is about 5% faster again.
 constexpr int kStripCols = 128;
 // Output columns per strip, at every depth: 112 columns are 7 whole 128-byte
 // lines, and strip s writes columns [112 s, 112 s + 112), so where rows start
@@ -289,6 +290,232 @@ __global__ __launch_bounds__(256) void jacobi_fused_kernel(double* __restrict__ 
     fused_tile<R, T, WRITE_B, true>(Aout, B, Ain, N, rbase, j0, rows, lane);
 }
 
+// ---- fused time steps over tall chunks -------------------------------------
+//
+// The same cascade as fused_tile, for chunks of many rows (R >= 48): a
+// wavefront runs down its chunk in a loop, so the 2H halo rows above and
+// below are loaded and recomputed once per chunk instead of once per 16 rows.
+// Level-0 rows sit in a ring of kRing registers (rows q-2 .. q+kAhead at
+// step q) and level L's last three rows in win[L-1][row % 3]; the loop body
+// is a block of kBlk steps starting at a multiple of kBlk, so every ring and
+// window slot is a compile-time index.  Steps 0 .. kWarm-1 (the levels
+// starting up) have compile-time step numbers.  A block runs the interior
+// form (no boundary code, buffer-resource addressing) when every row it
+// loads, computes or stores lies inside the grid and its stores stay inside
+// the chunk; otherwise, and for the chunk's last block, it runs the edge
+// form of fused_tile.  Each output is computed by the same operations in the
+// same order as in fused_tile, so the results are bit-identical.
+constexpr int kRing = kAhead + 3;
+constexpr int kBlk = 9;
+constexpr int kWarm = 2 * kBlk;
+constexpr int kMinStreamRows = 48;
+constexpr int kMaxStreamRows = 4096;
+static_assert(kBlk % kRing == 0 && kBlk % 3 == 0, "block steps keep the ring and window slots in place");
+
+struct StreamGeo {
+  double* Aout;
+  double* B;
+  const double* Ain;
+  int64_t N, rbase, j0, jl;
+  int rows, NR, pitch, voff;
+  bool col_in, own, x_out, y_out;
+};
+
+// Step q = QB + U of the cascade (QB = q0, a multiple of kBlk; WARM: q0 = 0
+// and the step number is the compile-time U).  Interior accesses are
+// relative to level-0 row `base` (base = 0 in the warm-up, q0 - H after).
+template <int T, bool WRITE_B, bool EDGE, bool WARM, int U>
+__device__ __forceinline__ void stream_step(const StreamGeo& g, int q0, v2d (&a)[kRing], v2d (&win)[2 * T][3]) {
+  constexpr int H = 2 * T;
+  const int q = WARM ? U : q0 + U;
+  const int base = WARM ? 0 : q0 - H;
+  const int64_t origin = (g.rbase + base) * g.N + g.j0;
+  const int span = (kWarm + kAhead + H + 2) * g.pitch;
+  {
+    const int k = q + kAhead;  // level-0 row into the slot of row q - 3
+    v2d v{0.0, 0.0};
+    if constexpr (!EDGE) {
+      const auto in = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(g.Ain + origin), 0, span, kRsrcWord3);
+      v = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(in, g.voff, (k - base) * g.pitch, 0));
+    } else {
+      const int64_t row = g.rbase + k;
+      if (g.col_in && row >= 0 && row < g.N && k < g.NR) v = *reinterpret_cast<const v2d*>(g.Ain + row * g.N + g.jl);
+    }
+    a[(U + kAhead) % kRing] = v;
+  }
+#pragma unroll
+  for (int L = 1; L <= H; ++L) {
+    if (WARM && U < 2 * L) continue;  // level L starts once level L-1 has three rows
+    v2d up, c, dn;
+    if (L == 1) {
+      up = a[(U + kRing - 2) % kRing];
+      c = a[(U + kRing - 1) % kRing];
+      dn = a[U % kRing];
+    } else {
+      up = win[L - 2][(U - L - 1 + 3 * kBlk) % 3];
+      c = win[L - 2][(U - L + 3 * kBlk) % 3];
+      dn = win[L - 2][(U - L + 1 + 3 * kBlk) % 3];
+    }
+    const double w = lane_shr_zero(c.y);
+    const double e = lane_shl_zero(c.x);
+    v2d r;
+    r.x = point(c.x, w, c.y, dn.x, up.x);
+    r.y = point(c.y, c.x, e, dn.y, up.y);
+    const int k = q - L;  // row of this level row, relative to rbase
+    if constexpr (EDGE) {
+      const int64_t gr = g.rbase + k;
+      const double* keep = (L % 2 == 1) ? g.B : g.Ain;  // the boundary this level keeps
+      if (gr == 0 || gr == g.N - 1) {
+        if (g.col_in) r = *reinterpret_cast<const v2d*>(keep + gr * g.N + g.jl);
+      } else if (gr > 0 && gr < g.N - 1) {
+        if (g.jl == 0) r.x = keep[gr * g.N];
+        if (g.jl + 1 == g.N - 1) r.y = keep[gr * g.N + g.N - 1];
+      }
+    }
+    win[L - 1][(U - L + 3 * kBlk) % 3] = r;
+    if (L == H || (WRITE_B && L == H - 1)) {
+      if constexpr (!EDGE) {
+        // Interior blocks store only rows inside the chunk (checked by the caller).
+        if (!WARM || U - L >= H) {
+          const auto out =
+              __builtin_amdgcn_make_buffer_rsrc((L == H ? g.Aout : g.B) + origin, 0, span, kRsrcWord3);
+          if (g.own)
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, r), out, g.voff, (k - base) * g.pitch,
+                                                   kStoreNT);
+        }
+      } else if (k >= H && k < H + g.rows) {
+        double* o = (L == H ? g.Aout : g.B) + (g.rbase + k) * g.N + g.jl;
+        if (g.x_out && g.y_out) {
+          *reinterpret_cast<v2d*>(o) = r;
+        } else {
+          if (g.x_out) o[0] = r.x;
+          if (g.y_out) o[1] = r.y;
+        }
+      }
+    }
+  }
+}
+
+template <int T, bool WRITE_B, bool EDGE, bool WARM, int U0, int U1>
+__device__ __forceinline__ void stream_steps(const StreamGeo& g, int q0, v2d (&a)[kRing], v2d (&win)[2 * T][3]) {
+  if constexpr (U0 < U1) {
+    asm volatile("" ::: "memory");
+    stream_step<T, WRITE_B, EDGE, WARM, U0>(g, q0, a, win);
+    stream_steps<T, WRITE_B, EDGE, WARM, U0 + 1, U1>(g, q0, a, win);
+  }
+}
+
+template <int T, bool EDGE>
+__device__ __forceinline__ void stream_first_loads(const StreamGeo& g, v2d (&a)[kRing]) {
+#pragma unroll
+  for (int k = 0; k <= kAhead + 1; ++k) {
+    v2d v{0.0, 0.0};
+    if constexpr (!EDGE) {
+      const auto in = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(g.Ain + g.rbase * g.N + g.j0), 0,
+                                                        (kWarm + kAhead + 2 * T + 2) * g.pitch, kRsrcWord3);
+      v = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(in, g.voff, k * g.pitch, 0));
+    } else {
+      const int64_t row = g.rbase + k;
+      if (g.col_in && row >= 0 && row < g.N && k < g.NR) v = *reinterpret_cast<const v2d*>(g.Ain + row * g.N + g.jl);
+    }
+    a[k % kRing] = v;
+  }
+}
+
+// The first and last strips (the grid's west and east columns) run the edge
+// form in every block, which waits on each boundary-value load; they take
+// chunks of kEdgeRows rows and the first workgroups of the launch, which the
+// dispatcher spreads over all XCDs, so they finish early.  The other strips
+// take chunks of R rows in the XCD-banded order.
+constexpr int kEdgeRows = 24;
+
+struct StreamGrid {
+  int strips, chunks, R;      // tall chunks of the inner strips 1 .. strips-2
+  int edge_strips, edge_chunks, edge_blocks;
+};
+
+template <int T, bool WRITE_B>
+__global__ __launch_bounds__(256) void jacobi_stream_kernel(double* __restrict__ Aout, double* __restrict__ B,
+                                                            const double* __restrict__ Ain, int64_t N,
+                                                            const StreamGrid sg) {
+  constexpr int H = 2 * T;
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned wave = threadIdx.x / kWave;
+  int strip, chunk, R;
+  if (blockIdx.x < unsigned(sg.edge_blocks)) {
+    const unsigned e = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + wave);
+    if (e >= unsigned(sg.edge_strips) * unsigned(sg.edge_chunks)) return;
+    const int side = int(e % unsigned(sg.edge_strips));
+    strip = side == 0 ? 0 : sg.strips - 1;
+    chunk = int(e / unsigned(sg.edge_strips));
+    R = kEdgeRows;
+  } else {
+    const unsigned inner = unsigned(sg.strips - sg.edge_strips);
+    const unsigned i = __builtin_amdgcn_readfirstlane(
+        logical_tile(blockIdx.x - sg.edge_blocks, gridDim.x - sg.edge_blocks) * (blockDim.x / kWave) + wave);
+    if (i >= inner * unsigned(sg.chunks)) return;
+    strip = 1 + int(i % inner);
+    chunk = int(i / inner);
+    R = sg.R;
+  }
+  StreamGeo g;
+  g.Aout = Aout;
+  g.B = B;
+  g.Ain = Ain;
+  g.N = N;
+  g.j0 = int64_t(strip) * kOwnCols - H;
+  const int64_t r0 = 1 + int64_t(chunk) * R;
+  int64_t rend = r0 + R;
+  if (rend > N - 1) rend = N - 1;
+  g.rows = int(rend - r0);
+  g.rbase = r0 - H;
+  g.NR = g.rows + 2 * H;
+  g.pitch = int(N) * int(sizeof(double));
+  g.voff = lane * 16;
+  g.jl = g.j0 + 2 * lane;
+  g.col_in = g.jl >= 0 && g.jl < N;
+  g.own = lane >= T && lane < T + kOwnCols / 2;
+  g.x_out = g.own && g.jl >= 1 && g.jl <= N - 2;
+  g.y_out = g.own && g.jl + 1 >= 1 && g.jl + 1 <= N - 2;
+  const bool strip_in = g.j0 >= 1 && g.j0 + kStripCols <= N - 1;
+
+  v2d a[kRing];
+  v2d win[H][3];
+  // Warm-up, steps 2 .. kWarm-1: interior when its rows (loads up to row
+  // kWarm-1+kAhead, stores up to row kWarm-H) lie inside the grid and the chunk.
+  if (strip_in && g.rbase >= 0 && g.rbase + kWarm + kAhead <= N - 1 && g.NR > kWarm + 1) {
+    stream_first_loads<T, false>(g, a);
+    stream_steps<T, WRITE_B, false, true, 2, kWarm>(g, 0, a, win);
+  } else {
+    stream_first_loads<T, true>(g, a);
+    stream_steps<T, WRITE_B, true, true, 2, kWarm>(g, 0, a, win);
+  }
+  // A block runs the interior form when its stores (A rows up to
+  // q0+kBlk-1-H, B rows one further) stay inside the chunk and its rows lie
+  // inside the grid.  Three loops, each with one form, so that the loop that
+  // carries the interior blocks sees only its own loads and stores in flight
+  // (the compiler's waits at a loop edge that joins both forms drained every
+  // block).
+  auto rows_in = [&](int q) {
+    return q + kBlk + 1 <= g.NR && g.rbase + q - H >= 1 && g.rbase + q + kBlk - 1 + kAhead <= N - 1;
+  };
+  int q0 = kWarm;
+  if (strip_in) {
+    while (q0 + kBlk + 1 <= g.NR && !rows_in(q0)) {
+      stream_steps<T, WRITE_B, true, false, 0, kBlk>(g, q0, a, win);
+      q0 += kBlk;
+    }
+    while (rows_in(q0)) {
+      stream_steps<T, WRITE_B, false, false, 0, kBlk>(g, q0, a, win);
+      q0 += kBlk;
+    }
+  }
+  while (q0 < g.NR) {
+    stream_steps<T, WRITE_B, true, false, 0, kBlk>(g, q0, a, win);
+    q0 += kBlk;
+  }
+}
+
 // Copies the boundary rows and columns of src into dst.
 __global__ __launch_bounds__(256) void boundary_copy_kernel(double* __restrict__ dst, const double* __restrict__ src,
                                                             int64_t N) {
@@ -317,7 +544,25 @@ int launch_fused(double* Aout, double* B, const double* Ain, int64_t N, bool wri
 }
 
 template <int T>
+int launch_stream(double* Aout, double* B, const double* Ain, int64_t N, int rows, bool write_b, hipStream_t s) {
+  const int64_t strips = ceil_div(N - 1, kOwnCols);
+  const int64_t chunks = ceil_div(N - 2, rows);
+  const int64_t edge_strips = strips < 2 ? strips : 2;
+  const int64_t edge_chunks = ceil_div(N - 2, kEdgeRows);
+  const int64_t edge_blocks = ceil_div(edge_strips * edge_chunks, 4);
+  const int64_t grid = edge_blocks + ceil_div((strips - edge_strips) * chunks, 4);
+  if (grid > INT32_MAX || strips > INT32_MAX || edge_chunks > INT32_MAX) return RPX_ERR_UNSUPPORTED;
+  const StreamGrid sg{int(strips), int(chunks), rows, int(edge_strips), int(edge_chunks), int(edge_blocks)};
+  if (write_b)
+    jacobi_stream_kernel<T, true><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, sg);
+  else
+    jacobi_stream_kernel<T, false><<<dim3(unsigned(grid)), dim3(256), 0, s>>>(Aout, B, Ain, N, sg);
+  return launch_status();
+}
+
+template <int T>
 int fused_rows(double* Aout, double* B, const double* Ain, int64_t N, int rows, bool write_b, hipStream_t s) {
+  if (rows >= kMinStreamRows) return launch_stream<T>(Aout, B, Ain, N, rows, write_b, s);
   switch (rows) {
     case 4: return launch_fused<4, T>(Aout, B, Ain, N, write_b, s);
     case 8: return launch_fused<8, T>(Aout, B, Ain, N, write_b, s);
@@ -343,12 +588,13 @@ int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows
 }
 
 constexpr int kDefaultRows = 16;
-constexpr int kDefaultFusedRows = 16;
+constexpr int kDefaultFusedRows = 99;  // streaming chunks (jacobi_stream_kernel)
 constexpr int kDefaultDepth = 3;  // time steps per fused launch
 constexpr int kMaxDepth = 3;
 bool fused_shape_ok(int rows, int depth) {
   if (depth < 1 || depth > kMaxDepth) return false;
-  return rows == 4 || rows == 8 || rows == 12 || rows == 16 || (rows == 32 && depth == 1);
+  return rows == 4 || rows == 8 || rows == 12 || rows == 16 || (rows == 32 && depth == 1) ||
+         (rows >= kMinStreamRows && rows <= kMaxStreamRows);
 }
 
 // Buffer offsets within a wavefront's rows are 32-bit: (R + 4T) * N * 8

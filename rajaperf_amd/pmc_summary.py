@@ -58,6 +58,10 @@ def _kernels():
         ("jacobi_fused_kernel<16, 2, false>", "jacobi2d_fused_depth2_rows16_skip_b_8192", a_read + interior),
         ("jacobi_fused_kernel<16, 3, false>", "jacobi2d_fused_depth3_rows16_skip_b_8192", a_read + interior),
         ("jacobi_fused_kernel<16, 2, true>", "jacobi2d_fused_depth2_rows16_8192", a_read + 2 * interior),
+        # The streaming kernel, timed only at chunks of 99 rows.
+        ("jacobi_stream_kernel<3, false>", "jacobi2d_fused_depth3_rows99_skip_b_8192", a_read + interior),
+        ("jacobi_stream_kernel<2, false>", "jacobi2d_fused_depth2_rows99_skip_b_8192", a_read + interior),
+        ("jacobi_stream_kernel<3, true>", "jacobi2d_fused_depth3_rows99_8192", a_read + 2 * interior),
         ("mass3dpa_kernel<4, 5>", "mass3dpa_262144_d4_q5", 8 * (3 * D ** 3 + Q ** 3) * NE),
         ("mass3dpa_naive_kernel<4, 5>", "mass3dpa_plain_262144_d4_q5", 8 * (3 * D ** 3 + Q ** 3) * NE),
         ("diffusion3dpa_kernel<4, 5, true>", "diffusion3dpa_262144_d4_q5", 8 * (3 * D ** 3 + 6 * Q ** 3) * NE),

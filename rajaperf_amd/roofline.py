@@ -124,9 +124,11 @@ def config2(rows, reps):
     # wavefront.  "B skipped": the launches of a run other than the last
     # store no B (each run's B is overwritten before it is read).
     skip = abi.RPX_TUNE_JACOBI_SKIP_B
-    for depth, rows_t, flags in ((1, 8, 0), (2, 8, 0), (3, 8, 0), (2, 16, 0), (3, 16, 0), (1, 8, skip), (2, 4, skip),
-                                 (2, 8, skip), (2, 12, skip), (2, 16, skip), (3, 4, skip), (3, 8, skip),
-                                 (3, 12, skip), (3, 16, skip)):
+    # Rows of 48 and more run the streaming kernel: a wavefront walks a chunk
+    # of that many rows (the first and last strips in chunks of 24).
+    for depth, rows_t, flags in ((1, 8, 0), (2, 8, 0), (3, 8, 0), (2, 16, 0), (3, 16, 0), (3, 99, 0), (1, 8, skip),
+                                 (2, 4, skip), (2, 8, skip), (2, 12, skip), (2, 16, skip), (2, 99, skip),
+                                 (3, 4, skip), (3, 8, skip), (3, 12, skip), (3, 16, skip), (3, 99, skip)):
         grids = [gA.ptr, gQ.ptr]
         t = abi.tuning(0, rows_t, 0, flags)
 
@@ -134,10 +136,11 @@ def config2(rows, reps):
             L.rpx_jacobi2d_fused_launch_f64(grids[1], gB.ptr, grids[0], N, depth, t, None)
             grids.reverse()
         ms = _time(launch, 4 * reps)
-        kind = f"rows {rows_t}" + (", B skipped" if flags else "")
+        kind = (f"rows {rows_t}" if rows_t < 48 else f"streaming chunks of {rows_t} rows") + \
+            (", B skipped" if flags else "")
         _row(rows, f"Polybench_JACOBI_2D(fused depth {depth}, {kind}, per launch)", 2, ms, 2 * depth * sweep)
     sb = L.rpx_jacobi2d_fused_scratch_bytes(N)
-    for depth, rows_t in ((2, 16), (3, 8), (3, 16)):
+    for depth, rows_t in ((2, 16), (3, 8), (3, 16), (3, 99)):
         t = abi.tuning(0, rows_t, 0, 0, depth)
         ms = _time(lambda: L.rpx_jacobi2d_fused_f64(gA.ptr, gB.ptr, N, 50, gQ.ptr, sb, t, None), max(1, reps // 2))
         _row(rows, f"Polybench_JACOBI_2D(50 time steps at depth {depth}, rows {rows_t}, per time step)", 2,
