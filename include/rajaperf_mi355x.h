@@ -281,9 +281,12 @@ size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
  * Aout != Ain, and Aout's boundary equal to Ain's (copy it once with
  * rpx_jacobi2d_copy_boundary_f64; the launch may rewrite boundary columns
  * with those same values); callers ping-pong Ain and Aout between launches.
- * tune->items_per_thread: output rows per wavefront tile, 4, 8, 12,
- * 16 (default), or 32 at depth 1.  Returns RPX_ERR_UNSUPPORTED for odd N,
- * N > 2^22 or an unsupported rows/depth pair. */
+ * tune->items_per_thread: output rows per wavefront, as a tile of 4, 8,
+ * 12 or 16 rows (32 at depth 1), or as a chunk of 48..4096 rows that the
+ * wavefront walks down in a loop (streaming).  Default: chunks of 99 rows
+ * for a launch that skips B, tiles of 16 rows for one that stores B.
+ * Returns RPX_ERR_UNSUPPORTED for odd N, N > 2^22 or an unsupported
+ * rows/depth pair. */
 int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain,
                                   int64_t N, int depth,
                                   const rpx_tuning* tune, void* stream);
