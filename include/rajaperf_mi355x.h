@@ -284,7 +284,8 @@ size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
  * tune->items_per_thread: output rows per wavefront, as a tile of 4, 8,
  * 12 or 16 rows (32 at depth 1), or as a chunk of 48..4096 rows that the
  * wavefront walks down in a loop (streaming).  Default: chunks of 99 rows
- * for a launch that skips B, tiles of 16 rows for one that stores B.
+ * for a launch that skips B on a grid of N >= 3072, tiles of 16 rows for
+ * one that stores B and on smaller grids.
  * Returns RPX_ERR_UNSUPPORTED for odd N, N > 2^22 or an unsupported
  * rows/depth pair. */
 int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain,

@@ -589,13 +589,18 @@ int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows
 
 constexpr int kDefaultRows = 16;
 // Default rows per wavefront: streaming chunks of 99 rows for the launches
-// that store no B, 16-row tiles for a launch that stores B (with B's stores
-// as well the tile kernel is faster: 0.321 against 0.337 ms per depth-2
-// launch at N = 8192).  An explicit `rows` applies to every launch.
+// that store no B on grids of N >= 3072, 16-row tiles otherwise.  With B's
+// stores as well the tile kernel is faster (0.321 against 0.337 ms per
+// depth-2 launch at N = 8192), and on smaller grids tall chunks leave too few
+// wavefronts (depth 3 at N = 1026 and 2050: 0.027 and 0.035 ms with tiles,
+// 0.039 and 0.046 ms at best with chunks; at N = 4098: 0.069 against
+// 0.059 ms).  An explicit `rows` applies to every launch.
 constexpr int kDefaultFusedRows = 99;
-constexpr int kDefaultFusedRowsWriteB = 16;
-inline int fused_rows_for(int rows_req, bool write_b) {
-  return rows_req ? rows_req : (write_b ? kDefaultFusedRowsWriteB : kDefaultFusedRows);
+constexpr int kDefaultFusedRowsTile = 16;
+constexpr int64_t kMinStreamN = 3072;
+inline int fused_rows_for(int rows_req, bool write_b, int64_t N) {
+  if (rows_req) return rows_req;
+  return (write_b || N < kMinStreamN) ? kDefaultFusedRowsTile : kDefaultFusedRows;
 }
 constexpr int kDefaultDepth = 3;  // time steps per fused launch
 constexpr int kMaxDepth = 3;
@@ -655,7 +660,7 @@ int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain, in
     return RPX_ERR_UNSUPPORTED;
   if (N < 3) return RPX_SUCCESS;
   const bool write_b = (tune_flags(tune) & RPX_TUNE_JACOBI_SKIP_B) == 0;
-  const int rows = fused_rows_for(tune_or(tune, &rpx_tuning::items_per_thread, 0), write_b);
+  const int rows = fused_rows_for(tune_or(tune, &rpx_tuning::items_per_thread, 0), write_b, N);
   if (!fused_shape_ok(rows, depth))
     return RPX_ERR_UNSUPPORTED;
   return fused_launch(Aout, B, Ain, N, rows, depth, write_b, as_stream(stream));
@@ -679,7 +684,8 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   if (scratch_bytes < rpx_jacobi2d_fused_scratch_bytes(N)) return RPX_ERR_SCRATCH_TOO_SMALL;
   const int rows_req = tune_or(tune, &rpx_tuning::items_per_thread, 0);
   const int depth = tune_or(tune, &rpx_tuning::depth, kDefaultDepth);
-  if (!fused_shape_ok(fused_rows_for(rows_req, false), depth) || !fused_shape_ok(fused_rows_for(rows_req, true), depth))
+  if (!fused_shape_ok(fused_rows_for(rows_req, false, N), depth) ||
+      !fused_shape_ok(fused_rows_for(rows_req, true, N), depth))
     return RPX_ERR_UNSUPPORTED;
 
   // Launch plan: depths summing to tsteps, an even number of launches so the
@@ -721,7 +727,7 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   auto go = [&](int d) {
     if (rc != RPX_SUCCESS || d == 0) return;
     const bool last = --left == 0;
-    rc = fused_launch(dst, B, src, N, fused_rows_for(rows_req, last), d, last, s);
+    rc = fused_launch(dst, B, src, N, fused_rows_for(rows_req, last, N), d, last, s);
     double* t = src;
     src = dst;
     dst = t;
