@@ -56,7 +56,7 @@ typedef struct rpx_tuning {
   int32_t grid_size;         /* workgroups; 0 = derived from the size       */
   int32_t flags;             /* RPX_TUNE_* bits                             */
   int32_t depth;             /* temporal blocking: JACOBI_2D time steps per
-                                fused launch (1..3)                          */
+                                fused launch (1..4)                          */
 } rpx_tuning;
 
 #define RPX_TUNE_CACHED            0x1 /* stream/DOT/REDUCE_SUM/SCAN: regular cached
@@ -267,7 +267,8 @@ int rpx_jacobi2d_sweep_f64(double* dst, const double* src, int64_t N,
 int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
                      const rpx_tuning* tune, void* stream);
 /* Same result with temporal blocking: each launch runs `depth` whole time
- * steps (2*depth sweeps, tune->depth, default 3) on register-resident tiles,
+ * steps (2*depth sweeps, tune->depth: 1..4, default 3) on register-resident
+ * tiles or chunks,
  * reading A once and writing only the final B and the new A, the new A going
  * to `scratch` (an N x N grid) and back, so no launch overwrites values a
  * neighbouring tile still reads.  Ends with the result in A and B, bit-equal
@@ -275,7 +276,7 @@ int rpx_jacobi2d_f64(double* A, double* B, int64_t N, int64_t tsteps,
  * N > 2^22 or pointers that are not 16-byte aligned. */
 size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
 /* The building block of rpx_jacobi2d_fused_f64: one launch advancing `depth`
- * time steps (1..3) from Ain into Aout, writing the final B (unless
+ * time steps (1..4) from Ain into Aout, writing the final B (unless
  * tune->flags has RPX_TUNE_JACOBI_SKIP_B: in a run of launches only the last
  * one needs to store B).  Requires even N, 16-byte aligned grids,
  * Aout != Ain, and Aout's boundary equal to Ain's (copy it once with
@@ -283,9 +284,10 @@ size_t rpx_jacobi2d_fused_scratch_bytes(int64_t N);
  * with those same values); callers ping-pong Ain and Aout between launches.
  * tune->items_per_thread: output rows per wavefront, as a tile of 4, 8,
  * 12 or 16 rows (32 at depth 1), or as a chunk of 48..4096 rows that the
- * wavefront walks down in a loop (streaming).  Default: chunks of 99 rows
- * for a launch that skips B on a grid of N >= 3072, tiles of 16 rows for
- * one that stores B and on smaller grids.
+ * wavefront walks down in a loop (streaming).  `depth` 4 runs streaming
+ * chunks only.  Default rows: at depth 4, chunks of 144; otherwise chunks of
+ * 99 rows for a launch that skips B on a grid of N >= 3072, tiles of 16
+ * rows for one that stores B and on smaller grids.
  * Returns RPX_ERR_UNSUPPORTED for odd N, N > 2^22 or an unsupported
  * rows/depth pair. */
 int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain,

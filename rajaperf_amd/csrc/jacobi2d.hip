@@ -439,6 +439,7 @@ __global__ __launch_bounds__(256) void jacobi_stream_kernel(double* __restrict__
                                                             const double* __restrict__ Ain, int64_t N,
                                                             const StreamGrid sg) {
   constexpr int H = 2 * T;
+  static_assert(2 * H <= kStripCols - kOwnCols, "halo columns on both sides of the owned ones");
   const int lane = threadIdx.x & (kWave - 1);
   const unsigned wave = threadIdx.x / kWave;
   int strip, chunk, R;
@@ -583,6 +584,8 @@ int fused_launch(double* Aout, double* B, const double* Ain, int64_t N, int rows
     case 1: return fused_rows<1>(Aout, B, Ain, N, rows, write_b, s);
     case 2: return fused_rows<2>(Aout, B, Ain, N, rows, write_b, s);
     case 3: return fused_rows<3>(Aout, B, Ain, N, rows, write_b, s);
+    case 4:  // streaming chunks only: the 16-row tile at depth 4 was slower than depth 3
+      return rows >= kMinStreamRows ? launch_stream<4>(Aout, B, Ain, N, rows, write_b, s) : RPX_ERR_UNSUPPORTED;
     default: return RPX_ERR_UNSUPPORTED;
   }
 }
@@ -595,16 +598,22 @@ constexpr int kDefaultRows = 16;
 // wavefronts (depth 3 at N = 1026 and 2050: 0.027 and 0.035 ms with tiles,
 // 0.039 and 0.046 ms at best with chunks; at N = 4098: 0.069 against
 // 0.059 ms).  An explicit `rows` applies to every launch.
+// Four time steps per launch (tune->depth = 4: streaming chunks only, 144
+// rows by default) are faster still per time step, 0.060 against 0.073 ms
+// at N = 8192; the default depth stays 3.
 constexpr int kDefaultFusedRows = 99;
+constexpr int kDefaultFusedRowsDepth4 = 144;
 constexpr int kDefaultFusedRowsTile = 16;
 constexpr int64_t kMinStreamN = 3072;
-inline int fused_rows_for(int rows_req, bool write_b, int64_t N) {
+inline int fused_rows_for(int rows_req, bool write_b, int64_t N, int depth) {
   if (rows_req) return rows_req;
+  if (depth == 4) return kDefaultFusedRowsDepth4;
   return (write_b || N < kMinStreamN) ? kDefaultFusedRowsTile : kDefaultFusedRows;
 }
-constexpr int kDefaultDepth = 3;  // time steps per fused launch
-constexpr int kMaxDepth = 3;
+inline int default_depth(int64_t) { return 3; }
+constexpr int kMaxDepth = 3;  // tiles; streaming chunks also run depth 4
 bool fused_shape_ok(int rows, int depth) {
+  if (depth == 4) return rows >= kMinStreamRows && rows <= kMaxStreamRows;
   if (depth < 1 || depth > kMaxDepth) return false;
   return rows == 4 || rows == 8 || rows == 12 || rows == 16 || (rows == 32 && depth == 1) ||
          (rows >= kMinStreamRows && rows <= kMaxStreamRows);
@@ -660,7 +669,7 @@ int rpx_jacobi2d_fused_launch_f64(double* Aout, double* B, const double* Ain, in
     return RPX_ERR_UNSUPPORTED;
   if (N < 3) return RPX_SUCCESS;
   const bool write_b = (tune_flags(tune) & RPX_TUNE_JACOBI_SKIP_B) == 0;
-  const int rows = fused_rows_for(tune_or(tune, &rpx_tuning::items_per_thread, 0), write_b, N);
+  const int rows = fused_rows_for(tune_or(tune, &rpx_tuning::items_per_thread, 0), write_b, N, depth);
   if (!fused_shape_ok(rows, depth))
     return RPX_ERR_UNSUPPORTED;
   return fused_launch(Aout, B, Ain, N, rows, depth, write_b, as_stream(stream));
@@ -683,9 +692,9 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   if (!scratch || scratch == A || scratch == B) return RPX_ERR_INVALID_ARGUMENT;
   if (scratch_bytes < rpx_jacobi2d_fused_scratch_bytes(N)) return RPX_ERR_SCRATCH_TOO_SMALL;
   const int rows_req = tune_or(tune, &rpx_tuning::items_per_thread, 0);
-  const int depth = tune_or(tune, &rpx_tuning::depth, kDefaultDepth);
-  if (!fused_shape_ok(fused_rows_for(rows_req, false, N), depth) ||
-      !fused_shape_ok(fused_rows_for(rows_req, true, N), depth))
+  const int depth = tune_or(tune, &rpx_tuning::depth, default_depth(N));
+  if (!fused_shape_ok(fused_rows_for(rows_req, false, N, depth), depth) ||
+      !fused_shape_ok(fused_rows_for(rows_req, true, N, depth), depth))
     return RPX_ERR_UNSUPPORTED;
 
   // Launch plan: depths summing to tsteps, an even number of launches so the
@@ -727,7 +736,7 @@ int rpx_jacobi2d_fused_f64(double* A, double* B, int64_t N, int64_t tsteps, void
   auto go = [&](int d) {
     if (rc != RPX_SUCCESS || d == 0) return;
     const bool last = --left == 0;
-    rc = fused_launch(dst, B, src, N, fused_rows_for(rows_req, last, N), d, last, s);
+    rc = fused_launch(dst, B, src, N, fused_rows_for(rows_req, last, N, d), d, last, s);
     double* t = src;
     src = dst;
     dst = t;

@@ -58,7 +58,9 @@ def _kernels():
         ("jacobi_fused_kernel<16, 2, false>", "jacobi2d_fused_depth2_rows16_skip_b_8192", a_read + interior),
         ("jacobi_fused_kernel<16, 3, false>", "jacobi2d_fused_depth3_rows16_skip_b_8192", a_read + interior),
         ("jacobi_fused_kernel<16, 2, true>", "jacobi2d_fused_depth2_rows16_8192", a_read + 2 * interior),
-        # The streaming kernel, timed only at chunks of 99 rows.
+        # The streaming kernel, timed at chunks of 99 rows (depths 2, 3) and 144 (depth 4).
+        ("jacobi_stream_kernel<4, false>", "jacobi2d_fused_depth4_rows144_skip_b_8192", a_read + interior),
+        ("jacobi_stream_kernel<4, true>", "jacobi2d_fused_depth4_rows144_8192", a_read + 2 * interior),
         ("jacobi_stream_kernel<3, false>", "jacobi2d_fused_depth3_rows99_skip_b_8192", a_read + interior),
         ("jacobi_stream_kernel<2, false>", "jacobi2d_fused_depth2_rows99_skip_b_8192", a_read + interior),
         ("jacobi_stream_kernel<3, true>", "jacobi2d_fused_depth3_rows99_8192", a_read + 2 * interior),

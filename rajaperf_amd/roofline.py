@@ -126,9 +126,10 @@ def config2(rows, reps):
     skip = abi.RPX_TUNE_JACOBI_SKIP_B
     # Rows of 48 and more run the streaming kernel: a wavefront walks a chunk
     # of that many rows (the first and last strips in chunks of 24).
-    for depth, rows_t, flags in ((1, 8, 0), (2, 8, 0), (3, 8, 0), (2, 16, 0), (3, 16, 0), (3, 99, 0), (1, 8, skip),
-                                 (2, 4, skip), (2, 8, skip), (2, 12, skip), (2, 16, skip), (2, 99, skip),
-                                 (3, 4, skip), (3, 8, skip), (3, 12, skip), (3, 16, skip), (3, 99, skip)):
+    for depth, rows_t, flags in ((1, 8, 0), (2, 8, 0), (3, 8, 0), (2, 16, 0), (3, 16, 0), (3, 99, 0), (4, 144, 0),
+                                 (1, 8, skip), (2, 4, skip), (2, 8, skip), (2, 12, skip), (2, 16, skip),
+                                 (2, 99, skip), (3, 4, skip), (3, 8, skip), (3, 12, skip), (3, 16, skip),
+                                 (3, 99, skip), (4, 144, skip)):
         grids = [gA.ptr, gQ.ptr]
         t = abi.tuning(0, rows_t, 0, flags)
 
@@ -140,7 +141,7 @@ def config2(rows, reps):
             (", B skipped" if flags else "")
         _row(rows, f"Polybench_JACOBI_2D(fused depth {depth}, {kind}, per launch)", 2, ms, 2 * depth * sweep)
     sb = L.rpx_jacobi2d_fused_scratch_bytes(N)
-    for depth, rows_t in ((2, 16), (3, 8), (3, 16), (3, 99)):
+    for depth, rows_t in ((2, 16), (3, 8), (3, 16), (3, 99), (4, 144)):
         t = abi.tuning(0, rows_t, 0, 0, depth)
         ms = _time(lambda: L.rpx_jacobi2d_fused_f64(gA.ptr, gB.ptr, N, 50, gQ.ptr, sb, t, None), max(1, reps // 2))
         _row(rows, f"Polybench_JACOBI_2D(50 time steps at depth {depth}, rows {rows_t}, per time step)", 2,

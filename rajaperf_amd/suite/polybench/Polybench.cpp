@@ -96,7 +96,7 @@ POLYBENCH_JACOBI_2D::POLYBENCH_JACOBI_2D(const RunParams& params) : KernelBase(P
   setVariantDefined(MI355X_Native);
 }
 void POLYBENCH_JACOBI_2D::setMI355XTuningDefinitions(VariantID vid) {
-  addVariantTuningName(vid, "default");    // three time steps per fused launch, 99-row streaming chunks
+  addVariantTuningName(vid, "default");    // three time steps per fused launch (99-row chunks from N = 3072)
   addVariantTuningName(vid, "depth_2");    // two time steps per fused launch
   addVariantTuningName(vid, "depth_1");    // one time step per fused launch
   addVariantTuningName(vid, "two_sweep");  // one launch per sweep, as the suite's kernels
@@ -144,8 +144,9 @@ void POLYBENCH_JACOBI_2D::runMI355XVariant(VariantID, size_t tune_idx) {
     // Trials advance copies of the initial grids, so A and B still start
     // from DATA_SETUP's values when the timed reps begin.
     std::vector<TuneCandidate> cands;
-    for (int depth = 1; depth <= 3; ++depth)
-      for (int rows : {8, 16, 54, 99}) {
+    for (int depth = 1; depth <= 4; ++depth)
+      for (int rows : {8, 16, 54, 99, 144}) {
+        if (depth == 4 && rows < 54) continue;  // depth 4 runs streaming chunks only
         rpx_tuning t = defaultTuning();
         t.depth = depth;
         t.items_per_thread = rows;

@@ -196,7 +196,8 @@ def test_jacobi_bit_exact(rpx, orc, rng, N, tsteps):
 @pytest.mark.parametrize("tsteps", [1, 2, 3, 5, 6])
 @pytest.mark.parametrize("rows,depth", [(4, 1), (8, 1), (16, 1), (32, 1), (4, 2), (8, 2), (12, 2), (16, 2),
                                         (4, 3), (8, 3), (12, 3), (16, 3),
-                                        (48, 1), (54, 2), (54, 3), (99, 3), (207, 3), (4096, 3)])
+                                        (48, 1), (54, 2), (54, 3), (99, 3), (207, 3), (4096, 3),
+                                        (54, 4), (144, 4)])
 def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows, depth):
     """Fused launches against the two-sweep oracle, bit for bit.  Rows of 48
     and more run the streaming kernel (a wavefront walks a tall chunk)."""
@@ -211,7 +212,8 @@ def test_jacobi_fused_bit_exact(rpx, orc, rng, N, tsteps, rows, depth):
     assert np.array_equal(dB.to_array(), B)
 
 
-@pytest.mark.parametrize("rows,depth", [(32, 2), (32, 3), (6, 1), (8, 4), (8, 0), (47, 3), (4097, 3), (64, 4)])
+@pytest.mark.parametrize("rows,depth", [(32, 2), (32, 3), (6, 1), (8, 4), (16, 4), (8, 0), (47, 3), (4097, 3),
+                                        (64, 5)])
 def test_jacobi_fused_unsupported_shapes(rpx, rows, depth):
     """Tile shapes without a kernel are refused before anything is launched."""
     N = 130
@@ -239,8 +241,8 @@ def test_jacobi_tunings(rpx, orc, rng, rows):
 
 
 @pytest.mark.parametrize("N", [6, 130, 1026])
-@pytest.mark.parametrize("depth", [1, 2, 3])
-@pytest.mark.parametrize("rows", [4, 8, 16, 54, 207])
+@pytest.mark.parametrize("depth,rows", [(d, r) for d in (1, 2, 3) for r in (4, 8, 16, 54, 207)] +
+                         [(4, 54), (4, 144)])
 def test_jacobi_launch_sequence_skip_b(rpx, orc, rng, N, depth, rows):
     """A run of fused launches where only the last stores B (RPX_TUNE_JACOBI_SKIP_B
     on the others) ends in the same A and B as the two-sweep oracle."""
